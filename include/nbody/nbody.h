@@ -1,0 +1,166 @@
+/*
+ * nbody.h -- C ABI of the MI355X N-body engine (libnbody.so).
+ *
+ * Drop-in for the reference's headless simulation stages.  The reference has no
+ * FFI: its "operator API" is the set of OpenCL kernels that CMyApp::Update
+ * (src/MyApp.cpp:550-656) enqueues in a fixed order, wired by
+ * CMyApp::SetKernelArgs (src/MyApp.cpp:245-325) and sized by
+ * CMyApp::SetKernelConfig (src/MyApp.cpp:71-83).  Each nb_stage_* entry point
+ * below replaces one of those enqueues; nb_sim_create/upload replace
+ * InitParticles/InitBodyAttributes (src/MyApp.cpp:216-243, 331-504).
+ *
+ * Conventions: plain pointers and sizes; host arrays are AoS float[3*N]
+ * positions/velocities and float[N] masses (the reference's cl_p/cl_v/cl_m
+ * host layout); no exceptions cross the ABI; every call returns nb_status.
+ * Stage calls are asynchronous on the simulation's HIP stream unless noted.
+ * One nb_sim is not internally locked; distinct handles may be used from
+ * distinct threads.
+ */
+#ifndef NBODY_NBODY_H
+#define NBODY_NBODY_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NB_ABI_VERSION 1
+
+typedef enum nb_status {
+    NB_OK = 0,
+    NB_E_INVALID_ARG = 1,    /* bad config or argument (e.g. eps <= 0, mass <= 0) */
+    NB_E_NO_DEVICE = 2,      /* no HIP device */
+    NB_E_HIP = 3,            /* a HIP runtime call failed; see nb_last_error_detail() */
+    NB_E_OUT_OF_MEMORY = 4,
+    NB_E_TREE_OVERFLOW = 5,  /* cell pool exhausted (reference: build_tree.cl:109-114) */
+    NB_E_DEPTH_OVERFLOW = 6, /* tree deeper than NB_MAX_LEVEL (reference: calculate_forces.cl:63-67) */
+    NB_E_NONFINITE = 7,      /* NaN/Inf position */
+    NB_E_STATE = 8,          /* stage called before its inputs exist */
+    NB_E_ADMISSION = 9       /* ensemble: request does not fit the HBM budget */
+} nb_status;
+
+/* Enum values are those of src/SimulationConfig.h:10-38. */
+enum { NB_LEAPFROG = 1, NB_EULER = 2 };
+enum { NB_SPHERE_POS = 0, NB_UNIFORM_POS = 1 };
+enum { NB_RANDOM_VEL = 0, NB_STARTING_OUT_VEL = 1, NB_STARTING_IN_VEL = 2, NB_FUNC_ZERO_VEL = 3,
+       NB_TANGENT_XZ_VEL = 4 };
+enum { NB_BARNES_HUT = 0, NB_BRUTE_FORCE_GLOBAL = 1, NB_BRUTE_FORCE_LOCAL = 2 };
+
+/* Rule sets.  REFERENCE reproduces the tree shape and opening rule of the
+ * reference kernels (root hop into slot 0, build_tree.cl:57-58; y half chosen by
+ * the x comparison, build_tree.hcl:42; threshold (s/2)^2/theta + eps,
+ * calculate_forces.cl:45-62).  STANDARD: cubic root, consistent halving,
+ * criterion s/d < theta. */
+enum { NB_RULES_REFERENCE = 0, NB_RULES_STANDARD = 1 };
+/* Brute-force laws.  PLUMMER: m*d/(|d|^2+eps^2)^1.5.  REFERENCE_UNIT: the
+ * reference brute-force law m*n/(|n|^2+e^2), n = normalize(d), e = 0.01 for
+ * BRUTE_FORCE_GLOBAL (brute_force_update.cl:54-58) and 0.1 for
+ * BRUTE_FORCE_LOCAL (brute_force_update.cl:114-120). */
+enum { NB_LAW_PLUMMER = 0, NB_LAW_REFERENCE_UNIT = 1 };
+/* Per-target accumulation: plain FP32 or Kahan-compensated FP32. */
+enum { NB_ACCUM_PLAIN = 0, NB_ACCUM_KAHAN = 1 };
+
+#define NB_MAX_LEVEL 64   /* deepest cell level (calculate_forces.hcl:1 MAX_DEPTH) */
+#define NB_PATH_STRIDE 72 /* bytes per body in nb_export_bodies paths */
+#define NB_NUM_STAGES 8
+
+/* Stage indices for nb_stage_times. */
+enum { NB_STAGE_BOUNDS = 0, NB_STAGE_RESET = 1, NB_STAGE_BUILD = 2, NB_STAGE_SUMMARIZE = 3,
+       NB_STAGE_SORT = 4, NB_STAGE_FORCE = 5, NB_STAGE_UPDATE = 6, NB_STAGE_BRUTE = 7 };
+
+/* Stepping parameters: SimulationConfig + BarnesHutConfig fields that reach the
+ * kernels (src/SimulationConfig.h:85-104, src/BarnesHutConfig.h:58-60), plus
+ * the engine's own choices.  nb_config_default() gives the reference defaults. */
+typedef struct nb_config {
+    int32_t num_bodies;        /* SimulationConfig::num_of_bodies (20000) */
+    int32_t algorithm;         /* NB_BARNES_HUT / NB_BRUTE_FORCE_GLOBAL / NB_BRUTE_FORCE_LOCAL */
+    int32_t method;            /* NB_EULER (default) or NB_LEAPFROG (kick-drift-kick) */
+    int32_t rules;             /* NB_RULES_REFERENCE (default) or NB_RULES_STANDARD */
+    int32_t law;               /* brute force: NB_LAW_REFERENCE_UNIT (default) or NB_LAW_PLUMMER */
+    int32_t accumulation;      /* NB_ACCUM_PLAIN (default) or NB_ACCUM_KAHAN */
+    int32_t max_octree_depth;  /* BarnesHutConfig::max_octree_depth (32); kept, not used by kernels */
+    int32_t cell_capacity;     /* 0: reference sizing max(2N, 1024*CUs) rounded up to 16 */
+    float gravitational_constant; /* 6.67e-11 */
+    float theta;               /* BarnesHutConfig::theta (1.5) */
+    float epsilon;             /* BarnesHutConfig::epsilon (0.01) */
+    float dt;                  /* time step (the reference derives it from frame time) */
+    int32_t collision;         /* SimulationConfig::collision; must be 0 (stage not built) */
+    int32_t use_graph;         /* nb_step replays a captured HIP graph (default 1) */
+    int32_t time_stages;       /* record per-stage HIP events (default 0) */
+    int32_t reserved[5];
+} nb_config;
+
+typedef struct nb_sim nb_sim;
+
+/* Tree summary after a build. */
+typedef struct nb_tree_info {
+    int32_t num_cells;   /* cells in use, root included */
+    int32_t max_depth;   /* max over bodies of the level of the cell holding them (>= 1) */
+    int32_t capacity;    /* cell pool size */
+    int32_t status;      /* nb_status of the last step's device work */
+} nb_tree_info;
+
+void nb_config_default(nb_config* cfg);
+const char* nb_status_string(nb_status s);
+const char* nb_last_error_detail(void);
+int32_t nb_abi_version(void);
+/* Device description: compute units and name (name may be NULL). */
+nb_status nb_device_info(int32_t* compute_units, char* name, size_t name_len);
+/* Cell pool size the reference would allocate for n bodies (src/MyApp.cpp:71-83). */
+int32_t nb_reference_cell_capacity(int32_t n, int32_t compute_units);
+
+/* Create/destroy.  `stream` is a hipStream_t or NULL (the library creates one). */
+nb_status nb_sim_create(const nb_config* cfg, void* stream, nb_sim** out);
+nb_status nb_sim_destroy(nb_sim* sim);
+/* Host -> device state (InitBodyAttributes' writes, src/MyApp.cpp:357,416,489).
+ * vel may be NULL (zero).  Synchronous. */
+nb_status nb_sim_upload(nb_sim* sim, const float* pos_xyz, const float* vel_xyz, const float* mass);
+/* Device -> host; any pointer may be NULL.  Synchronous. */
+nb_status nb_sim_download(nb_sim* sim, float* pos_xyz, float* vel_xyz, float* acc_xyz);
+nb_status nb_sim_sync(nb_sim* sim);
+/* Status of device work queued so far (synchronises). */
+nb_status nb_sim_status(nb_sim* sim);
+nb_status nb_sim_clear_status(nb_sim* sim);
+
+/* Stages, in the order CMyApp::Update enqueues them (src/MyApp.cpp:583-598). */
+/* hybrid_reduce_root + parallel_reduce_root (kernels/boundary_reduce.cl:6-159);
+ * box (nullable) receives {minx,miny,minz,maxx,maxy,maxz} (synchronises if set). */
+nb_status nb_stage_bounds(nb_sim* sim, float* box6);
+/* init (kernels/init.cl:2-36). */
+nb_status nb_stage_tree_reset(nb_sim* sim);
+/* build_tree (kernels/build_tree.cl:6-165). */
+nb_status nb_stage_tree_build(nb_sim* sim);
+/* saturate_tree (kernels/saturate_tree.cl:6-146). */
+nb_status nb_stage_summarize(nb_sim* sim);
+/* sort (kernels/sort.cl, empty in the reference; intent at calculate_forces.cl:148). */
+nb_status nb_stage_sort(nb_sim* sim);
+/* calculate_force_local traversal (kernels/calculate_forces.cl:44-148); writes acc. */
+nb_status nb_stage_force(nb_sim* sim);
+/* Integrator (calculate_forces.cl:150-182): Euler v+=a dt, x+=v dt; under
+ * NB_LEAPFROG the closing kick.  Also releases the tree. */
+nb_status nb_stage_update(nb_sim* sim);
+/* update / update_local all-pairs force (kernels/brute_force_update.cl:3-228); writes acc. */
+nb_status nb_stage_brute_force(nb_sim* sim);
+/* nsteps whole steps (BH: bounds..update; BF: brute force + update). */
+nb_status nb_step(nb_sim* sim, int32_t nsteps);
+
+/* Tree inspection (synchronises).  Canonical forms: bodies in tree order with
+ * their octant paths (NB_PATH_STRIDE bytes each, unused bytes 0xff), cells in
+ * depth-first preorder.  Any pointer may be NULL. */
+nb_status nb_tree_info_get(nb_sim* sim, nb_tree_info* info);
+nb_status nb_export_bodies(nb_sim* sim, int32_t* sorted, uint8_t* paths, int32_t* path_len);
+nb_status nb_export_cells(nb_sim* sim, int32_t* level, int32_t* slot, int32_t* count, float* com4);
+/* Opening thresholds per level (NB_MAX_LEVEL+1 floats) of the current box. */
+nb_status nb_export_dq(nb_sim* sim, float* dq);
+/* Interactions the last nb_stage_force performed, summed over bodies (synchronises). */
+nb_status nb_force_interactions(nb_sim* sim, int64_t* total);
+
+/* Per-stage milliseconds of the last step (needs cfg.time_stages). */
+nb_status nb_stage_times(nb_sim* sim, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

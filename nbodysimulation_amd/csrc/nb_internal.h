@@ -1,0 +1,68 @@
+// nb_internal.h -- device data layout shared by the HIP kernels and the host engine.
+//
+// Every kernel is written for a batch of independent systems ("ensemble"); a
+// single simulation is a batch of one.  Per-system constants live in SysDesc,
+// per-system mutable device state in SysState, and launches are described by
+// block tables (one BlkWork per workgroup) built on the host.
+//
+// HBM layout (all systems concatenated):
+//   node4  float4[sum(n_s + cap_s)]  bodies [node_off, node_off+n) as (x,y,z,m),
+//                                    cells  [node_off+n, node_off+n+cap) as (com, mass)
+//   vel4   float4[sum n_s]           velocity (w unused)
+//   acc4   float4[sum n_s]           acceleration (w = interactions of the last force pass)
+//   sorted int[sum n_s]              local body ids in tree order
+//   child  int[8 * sum cap_s]        per cell 8 slots: -1 empty, -2 locked,
+//                                    [0,n) body, n+ci cell ci (local numbering)
+//   cparent, cpend, ccount, cstart   int[sum cap_s]
+#pragma once
+#include <stdint.h>
+
+namespace nb {
+
+constexpr int kMaxLevel = 64;          // deepest cell level (root = 0)
+constexpr int kDqLen = kMaxLevel + 1;  // opening thresholds per level
+constexpr int kEmpty = -1;
+constexpr int kLocked = -2;
+
+// status bits (SysState::status)
+constexpr int kStTreeOverflow = 1;
+constexpr int kStDepthOverflow = 2;
+constexpr int kStNonfinite = 4;
+
+struct SysDesc {
+    int n;          // bodies
+    int cap;        // cell capacity
+    int node_off;   // offset into node4
+    int body_off;   // offset into vel4/acc4/sorted
+    int cell_off;   // offset into cell arrays (child uses 8*cell_off)
+    int rules;      // 0 reference, 1 standard
+    int algo;       // 0 BH, 1 BF global, 2 BF local
+    int law;        // 0 plummer, 1 reference unit
+    int accum;      // 0 plain, 1 kahan
+    int method;     // 1 leapfrog, 2 euler
+    float G, theta, eps, dt, ueps;
+    int bbox_first;   // first global block of this system in the bounds table
+    int bbox_blocks;  // number of bounds blocks of this system
+    int pad[3];
+};
+
+struct SysState {
+    float box[6];
+    int status;
+    int ncells;
+    int max_depth;
+    int ticket;     // bounds last-block ticket (self-resetting)
+    int step;
+    int pad0;
+    float dq[kDqLen];
+    int pad1[3];
+};
+
+struct BlkWork {
+    int sys;
+    int begin;
+    int end;
+    int pad;
+};
+
+}  // namespace nb

@@ -1,0 +1,688 @@
+// nb_kernels.hip -- gfx950 kernels of the N-body stage pipeline.
+//
+// Stages (reference kernel each one replaces):
+//   k_bounds      hybrid_reduce_root + parallel_reduce_root  kernels/boundary_reduce.cl:6-159
+//   k_tree_clear  init (child reset)                         kernels/init.cl:2-36
+//   k_tree_init   init (root, bottom)                        kernels/init.cl:22-34
+//   k_build       build_tree                                 kernels/build_tree.cl:6-165
+//   k_summarize   saturate_tree                              kernels/saturate_tree.cl:6-146
+//   k_sort        sort (empty in the reference)              kernels/sort.cl
+//   k_force_bh    calculate_force_local traversal            kernels/calculate_forces.cl:44-148
+//   k_brute       update / update_local all-pairs            kernels/brute_force_update.cl:3-228
+//   k_update      integrator branch                          kernels/calculate_forces.cl:150-182
+//
+// Memory-model notes (gfx950 has one L2 per XCD):
+//   * Data that one workgroup writes and another reads inside the same kernel is
+//     accessed with agent-scope atomics (sc1 loads/stores).  The tree build relies
+//     only on compare-and-swap for correctness: a stale read of a child slot can
+//     only return an older value of that slot, which makes the CAS fail and the
+//     body retry.  Slots of unused cells are -1 before a build starts, so a stale
+//     read of a freshly published cell can only be -1.
+//   * A lock holder publishes a new subtree with one release store.
+//   * No lane ever waits inside an iteration for another lane of its own wave:
+//     every retry loop makes one attempt per iteration.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "nb_internal.h"
+
+namespace nb {
+
+// ---------------------------------------------------------------------------
+// helpers
+
+__device__ __forceinline__ int ld_c(const int* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float ld_cf(const float* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_c(int* p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_cf(float* p, float v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_rel(int* p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool cas_c(int* p, int expect, int desired) {
+    return __hip_atomic_compare_exchange_strong(p, &expect, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// DPP row reduction (16 lanes) followed by a read of the four row results:
+// every lane returns the wave-wide min / max.
+template <int CTRL>
+__device__ __forceinline__ float dppc(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float wave_min(float v) {
+    v = fminf(v, dppc<0xB1>(v));   // quad_perm [1,0,3,2]
+    v = fminf(v, dppc<0x4E>(v));   // quad_perm [2,3,0,1]
+    v = fminf(v, dppc<0x124>(v));  // row_ror:4
+    v = fminf(v, dppc<0x128>(v));  // row_ror:8
+    float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+    float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+    float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+    float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+    return fminf(fminf(r0, r1), fminf(r2, r3));
+}
+__device__ __forceinline__ float wave_max(float v) {
+    v = fmaxf(v, dppc<0xB1>(v));
+    v = fmaxf(v, dppc<0x4E>(v));
+    v = fmaxf(v, dppc<0x124>(v));
+    v = fmaxf(v, dppc<0x128>(v));
+    float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+    float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+    float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+    float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+    return fmaxf(fmaxf(r0, r1), fmaxf(r2, r3));
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+    for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Root cell box.  REFERENCE: the bounding box (build_tree.cl:53-54).  STANDARD:
+// cube of half-edge r = half the largest extent about the box centre.
+__device__ __forceinline__ void root_box(int rules, const float* box, float& lx, float& ly, float& lz,
+                                         float& hx, float& hy, float& hz) {
+    if (rules == 0) {
+        lx = box[0]; ly = box[1]; lz = box[2]; hx = box[3]; hy = box[4]; hz = box[5];
+        return;
+    }
+    float r = 0.5f * fmaxf(fmaxf(box[3] - box[0], box[4] - box[1]), box[5] - box[2]);
+    float cx = (box[0] + box[3]) * 0.5f, cy = (box[1] + box[4]) * 0.5f, cz = (box[2] + box[5]) * 0.5f;
+    lx = cx - r; hx = cx + r;
+    ly = cy - r; hy = cy + r;
+    lz = cz - r; hz = cz + r;
+}
+
+// Octant 4*[y<=cy] + 2*[x<=cx] + [z<=cz] (calculateOctantIdx, build_tree.hcl:14-29,
+// ties to the lower half) and the halving that follows (adjustBoundaryValues,
+// build_tree.hcl:31-54; REFERENCE rules halve y by the x comparison, :42).
+// Under REFERENCE rules level 0 is the root's single hop into slot 0.
+__device__ __forceinline__ int level_digit(int rules, int lvl, float px, float py, float pz, float& lx,
+                                           float& ly, float& lz, float& hx, float& hy, float& hz) {
+    if (rules == 0 && lvl == 0) return 0;
+    float cx = (lx + hx) * 0.5f, cy = (ly + hy) * 0.5f, cz = (lz + hz) * 0.5f;
+    bool gx = px > cx, gy = py > cy, gz = pz > cz;
+    int d = (gy ? 0 : 4) + (gx ? 0 : 2) + (gz ? 0 : 1);
+    if (gx) lx = cx; else hx = cx;
+    bool ys = rules == 0 ? gx : gy;
+    if (ys) ly = cy; else hy = cy;
+    if (gz) lz = cz; else hz = cz;
+    return d;
+}
+
+__device__ __forceinline__ bool finite3(float4 p) {
+    return isfinite(p.x) && isfinite(p.y) && isfinite(p.z);
+}
+
+// ---------------------------------------------------------------------------
+// bounds: exact min/max over all bodies of each system, canonical +0, plus the
+// per-level opening thresholds of calculate_forces.cl:45-62.  One pass: each
+// workgroup reduces its range (DPP inside a wave, LDS across waves), the last
+// workgroup of a system (ticket) reduces the partials.
+
+__global__ __launch_bounds__(256) void k_bounds(const SysDesc* __restrict__ sd, SysState* st,
+                                                const BlkWork* __restrict__ work,
+                                                const float4* __restrict__ node4, float* partial) {
+    __shared__ float red[4][7];
+    __shared__ int is_last;
+    const BlkWork w = work[blockIdx.x];
+    const SysDesc d = sd[w.sys];
+    SysState* S = st + w.sys;
+    const float4* nodes = node4 + d.node_off;
+    float lo0 = INFINITY, lo1 = INFINITY, lo2 = INFINITY;
+    float hi0 = -INFINITY, hi1 = -INFINITY, hi2 = -INFINITY;
+    float bad = 0.0f;
+    for (int i = w.begin + threadIdx.x; i < w.end; i += blockDim.x) {
+        float4 p = nodes[i];
+        if (finite3(p)) {
+            lo0 = fminf(lo0, p.x); lo1 = fminf(lo1, p.y); lo2 = fminf(lo2, p.z);
+            hi0 = fmaxf(hi0, p.x); hi1 = fmaxf(hi1, p.y); hi2 = fmaxf(hi2, p.z);
+        } else {
+            bad = 1.0f;
+        }
+    }
+    lo0 = wave_min(lo0); lo1 = wave_min(lo1); lo2 = wave_min(lo2);
+    hi0 = wave_max(hi0); hi1 = wave_max(hi1); hi2 = wave_max(hi2);
+    bad = wave_max(bad);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+        red[wave][0] = lo0; red[wave][1] = lo1; red[wave][2] = lo2;
+        red[wave][3] = hi0; red[wave][4] = hi1; red[wave][5] = hi2; red[wave][6] = bad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r[7];
+        for (int k = 0; k < 7; ++k) r[k] = red[0][k];
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) {
+            for (int k = 0; k < 3; ++k) r[k] = fminf(r[k], red[q][k]);
+            for (int k = 3; k < 7; ++k) r[k] = fmaxf(r[k], red[q][k]);
+        }
+        for (int k = 0; k < 7; ++k) st_cf(partial + 7 * (size_t)blockIdx.x + k, r[k]);
+        int t = __hip_atomic_fetch_add(&S->ticket, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        is_last = (t == d.bbox_blocks - 1);
+    }
+    __syncthreads();
+    if (!is_last || threadIdx.x != 0) return;
+    float r[7];
+    for (int k = 0; k < 3; ++k) r[k] = INFINITY;
+    for (int k = 3; k < 6; ++k) r[k] = -INFINITY;
+    r[6] = 0.0f;
+    for (int q = 0; q < d.bbox_blocks; ++q) {
+        const float* pp = partial + 7 * (size_t)(d.bbox_first + q);
+        for (int k = 0; k < 3; ++k) r[k] = fminf(r[k], ld_cf(pp + k));
+        for (int k = 3; k < 7; ++k) r[k] = fmaxf(r[k], ld_cf(pp + k));
+    }
+    for (int k = 0; k < 6; ++k) S->box[k] = r[k] + 0.0f;
+    if (r[6] != 0.0f) atomicOr(&S->status, kStNonfinite);
+    // opening thresholds per level (calculate_forces.cl:45-62 for REFERENCE rules)
+    float rad = 0.5f * fmaxf(fmaxf(S->box[3] - S->box[0], S->box[4] - S->box[1]), S->box[5] - S->box[2]);
+    if (d.rules == 0) {
+        float q = d.theta > 0.0f ? rad * rad / d.theta : rad * rad;
+        for (int l = 0; l < kDqLen; ++l) {
+            if (l) q = 0.25f * q;
+            S->dq[l] = q + d.eps;
+        }
+    } else if (d.theta > 0.0f) {
+        float s = rad / d.theta;
+        float e2 = d.eps * d.eps;
+        float q = s * s;
+        for (int l = 0; l < kDqLen; ++l) {
+            if (l) q = 0.25f * q;
+            S->dq[l] = q + e2;
+        }
+    } else {
+        for (int l = 0; l < kDqLen; ++l) S->dq[l] = INFINITY;
+    }
+    S->ticket = 0;
+}
+
+// ---------------------------------------------------------------------------
+// reset: clear the slots of the cells the previous build used (so every unused
+// cell holds -1 slots), then re-seed the root.
+
+__global__ __launch_bounds__(256) void k_tree_clear(const SysDesc* __restrict__ sd, const SysState* __restrict__ st,
+                                                    const BlkWork* __restrict__ work, int* child, int* cpend) {
+    const BlkWork w = work[blockIdx.x];
+    const SysDesc d = sd[w.sys];
+    const int used = min(st[w.sys].ncells, d.cap);
+    const int ci = w.begin + threadIdx.x;
+    if (ci >= w.end || ci >= used) return;
+    int4* c4 = reinterpret_cast<int4*>(child + 8 * ((size_t)d.cell_off + ci));
+    const int4 m1 = make_int4(-1, -1, -1, -1);
+    c4[0] = m1;
+    c4[1] = m1;
+    cpend[d.cell_off + ci] = 0;
+}
+
+__global__ void k_tree_init(int nsys, SysState* st) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys) return;
+    st[s].ncells = 1;
+    st[s].max_depth = 1;  // build_tree.cl:28
+}
+
+// ---------------------------------------------------------------------------
+// build: lock-free insertion (build_tree.cl:42-161), one body per lane.
+// A lane that finds its slot empty CASes its body in; a lane that finds a body
+// locks the slot (CAS to -2), computes how many cells it takes to separate the
+// two bodies, allocates them in one atomicAdd, writes the chain, and publishes
+// it with a release store.  A lane that finds the slot locked retries on its
+// next iteration.
+
+__global__ __launch_bounds__(256) void k_build(const SysDesc* __restrict__ sd, SysState* st,
+                                               const BlkWork* __restrict__ work,
+                                               const float4* __restrict__ node4, int* child, int* cparent,
+                                               int* cpend) {
+    const BlkWork w = work[blockIdx.x];
+    const SysDesc d = sd[w.sys];
+    SysState* S = st + w.sys;
+    if (S->status) return;
+    const int n = d.n;
+    const int b = w.begin + threadIdx.x;
+    bool active = b < w.end;
+    int* C = child + 8 * (size_t)d.cell_off;
+    int* par = cparent + d.cell_off;
+    int* pend = cpend + d.cell_off;
+    float rlx, rly, rlz, rhx, rhy, rhz;
+    root_box(d.rules, S->box, rlx, rly, rlz, rhx, rhy, rhz);
+    float px = 0.f, py = 0.f, pz = 0.f;
+    if (active) {
+        float4 p = node4[d.node_off + b];
+        px = p.x; py = p.y; pz = p.z;
+    }
+    float lx = rlx, ly = rly, lz = rlz, hx = rhx, hy = rhy, hz = rhz;
+    int node = 0, lvl = 0;
+    int dg = level_digit(d.rules, 0, px, py, pz, lx, ly, lz, hx, hy, hz);
+    int depth = 1;
+    while (active) {
+        int slot = 8 * node + dg;
+        int v = ld_c(C + slot);
+        while (v >= n) {
+            node = v - n;
+            ++lvl;
+            dg = level_digit(d.rules, lvl, px, py, pz, lx, ly, lz, hx, hy, hz);
+            slot = 8 * node + dg;
+            v = ld_c(C + slot);
+        }
+        if (v == kEmpty) {
+            if (cas_c(C + slot, kEmpty, b)) {
+                depth = max(depth, lvl);
+                active = false;
+            }
+        } else if (v >= 0) {
+            if (cas_c(C + slot, v, kLocked)) {
+                const float4 q4 = node4[d.node_off + v];
+                // count the cells needed to separate b and v
+                float ax = lx, ay = ly, az = lz, bx = hx, by = hy, bz = hz;
+                float qx0 = lx, qy0 = ly, qz0 = lz, qx1 = hx, qy1 = hy, qz1 = hz;
+                int k = 0, l = lvl, db, dq;
+                bool deep = false;
+                do {
+                    ++l;
+                    if (l > kMaxLevel) { deep = true; break; }
+                    db = level_digit(d.rules, l, px, py, pz, ax, ay, az, bx, by, bz);
+                    dq = level_digit(d.rules, l, q4.x, q4.y, q4.z, qx0, qy0, qz0, qx1, qy1, qz1);
+                    ++k;
+                } while (db == dq);
+                int err = deep ? kStDepthOverflow : 0;
+                int base = 0;
+                if (!err) {
+                    base = atomicAdd(&S->ncells, k);
+                    if (base + k > d.cap) err = kStTreeOverflow;
+                }
+                if (err) {
+                    atomicOr(&S->status, err);
+                    st_c(C + slot, v);  // unlock, leave the old body in place
+                    active = false;
+                } else {
+                    ax = lx; ay = ly; az = lz; bx = hx; by = hy; bz = hz;
+                    qx0 = lx; qy0 = ly; qz0 = lz; qx1 = hx; qy1 = hy; qz1 = hz;
+                    int parent = node;
+                    for (int i = 0; i < k; ++i) {
+                        const int c = base + i;
+                        const int l2 = lvl + 1 + i;
+                        const int db2 = level_digit(d.rules, l2, px, py, pz, ax, ay, az, bx, by, bz);
+                        const int dq2 = level_digit(d.rules, l2, q4.x, q4.y, q4.z, qx0, qy0, qz0, qx1, qy1, qz1);
+                        par[c] = parent;
+                        pend[c] = (i < k - 1) ? 1 : 0;
+                        if (i < k - 1) {
+                            st_c(C + 8 * c + db2, n + c + 1);
+                        } else {
+                            st_c(C + 8 * c + db2, b);
+                            st_c(C + 8 * c + dq2, v);
+                        }
+                        parent = c;
+                    }
+                    atomicAdd(pend + node, 1);  // the slot now holds a cell instead of a body
+                    st_rel(C + slot, n + base);
+                    depth = max(depth, lvl + k);
+                    active = false;
+                }
+            }
+        } else {
+            __builtin_amdgcn_s_sleep(1);
+            if (ld_c(&S->status)) active = false;
+        }
+    }
+    depth = wave_max_i(depth);
+    if ((threadIdx.x & 63) == 0) atomicMax(&S->max_depth, depth);
+}
+
+// ---------------------------------------------------------------------------
+// summarize: bottom-up mass, centre of mass and body count (saturate_tree.cl:6-146)
+// without spin-waits.  A lane starts at a cell whose children are all bodies;
+// after finishing a cell it decrements the parent's count of unfinished child
+// cells, and the lane that brings it to zero goes on with the parent.  Children
+// are summed in slot order (deterministic; the reference's late children are
+// summed LIFO, saturate_tree.cl:96-120): mass += m_k, com = fmaf(x_k, m_k, com),
+// then com * (1/mass).
+
+__global__ __launch_bounds__(256) void k_summarize(const SysDesc* __restrict__ sd, const SysState* __restrict__ st,
+                                                   const BlkWork* __restrict__ work, float4* node4,
+                                                   const int* __restrict__ child, const int* __restrict__ cparent,
+                                                   int* cpend, int* ccount, int* cstart) {
+    const BlkWork w = work[blockIdx.x];
+    const SysDesc d = sd[w.sys];
+    const SysState* S = st + w.sys;
+    if (S->status) return;
+    const int used = min(S->ncells, d.cap);
+    const int ci = w.begin + threadIdx.x;
+    if (ci >= w.end || ci >= used) return;
+    const int n = d.n;
+    const int* C = child + 8 * (size_t)d.cell_off;
+    {
+        const int4* c4 = reinterpret_cast<const int4*>(C + 8 * ci);
+        int4 a = c4[0], bq = c4[1];
+        int mx = max(max(max(a.x, a.y), max(a.z, a.w)), max(max(bq.x, bq.y), max(bq.z, bq.w)));
+        if (mx >= n) return;  // has child cells: its last finished child will handle it
+    }
+    float4* nodes = node4 + d.node_off;
+    int* cnt_base = ccount + d.cell_off;
+    int c = ci;
+    for (;;) {
+        const int4* c4 = reinterpret_cast<const int4*>(C + 8 * c);
+        int4 a = c4[0], bq = c4[1];
+        int vs[8] = {a.x, a.y, a.z, a.w, bq.x, bq.y, bq.z, bq.w};
+        float m = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+        int cnt = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int v = vs[k];
+            if (v < 0) continue;
+            float x, y, z, wm;
+            if (v < n) {
+                float4 t = nodes[v];
+                x = t.x; y = t.y; z = t.z; wm = t.w;
+                cnt += 1;
+            } else {
+                const float* f = reinterpret_cast<const float*>(nodes + v);
+                x = ld_cf(f); y = ld_cf(f + 1); z = ld_cf(f + 2); wm = ld_cf(f + 3);
+                cnt += ld_c(cnt_base + (v - n));
+            }
+            m = m + wm;
+            cx = fmaf(x, wm, cx);
+            cy = fmaf(y, wm, cy);
+            cz = fmaf(z, wm, cz);
+        }
+        const float inv = 1.0f / m;
+        float* o = reinterpret_cast<float*>(nodes + n + c);
+        st_cf(o, cx * inv);
+        st_cf(o + 1, cy * inv);
+        st_cf(o + 2, cz * inv);
+        st_cf(o + 3, m);
+        st_c(cnt_base + c, cnt);
+        cstart[d.cell_off + c] = -1;
+        if (c == 0) break;
+        const int p = cparent[d.cell_off + c];
+        const int old = __hip_atomic_fetch_add(cpend + d.cell_off + p, -1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old != 1) break;
+        c = p;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// sort: tree-order placement (Burtscher & Pingali), top-down.  Every lane walks
+// its cells in increasing cell number; a cell's start offset is published by
+// its parent, which always has a smaller number.  The grid is sized to be fully
+// resident, so the lowest unfinished cell can always proceed.
+
+__global__ __launch_bounds__(256) void k_sort(const SysDesc* __restrict__ sd, const SysState* __restrict__ st,
+                                              int nsys, int total_cap, const int* __restrict__ child,
+                                              const int* __restrict__ ccount, int* cstart, int* sorted) {
+    const int T = gridDim.x * blockDim.x;
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    int s = 0;
+    SysDesc d = sd[0];
+    int used = min(st[0].ncells, d.cap);
+    int bad = st[0].status;
+    while (g < total_cap) {
+        while (g >= d.cell_off + d.cap && s + 1 < nsys) {
+            ++s;
+            d = sd[s];
+            used = min(st[s].ncells, d.cap);
+            bad = st[s].status;
+        }
+        const int ci = g - d.cell_off;
+        if (bad || ci >= used) {
+            g += T;
+            continue;
+        }
+        int start = ci == 0 ? 0 : ld_c(cstart + g);
+        if (start < 0) {
+            __builtin_amdgcn_s_sleep(2);
+            continue;
+        }
+        const int4* c4 = reinterpret_cast<const int4*>(child + 8 * (size_t)g);
+        int4 a = c4[0], bq = c4[1];
+        int vs[8] = {a.x, a.y, a.z, a.w, bq.x, bq.y, bq.z, bq.w};
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int v = vs[k];
+            if (v < 0) continue;
+            if (v < d.n) {
+                sorted[d.body_off + start] = v;
+                start += 1;
+            } else {
+                const int cg = d.cell_off + (v - d.n);
+                st_c(cstart + cg, start);
+                start += ccount[cg];
+            }
+        }
+        g += T;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// force (Barnes-Hut): one wave per 64 consecutive bodies in tree order, one
+// shared traversal per wave.  A body child is always taken; a cell child at
+// parent level l is taken for the whole wave iff every lane has
+// eps^2+|d|^2 >= dq[l] (the wave-wide form of work_group_all,
+// calculate_forces.cl:122), otherwise pushed.  The stack lives in LDS; node
+// reads are wave-uniform (scalar loads).  acc = G * sum d*m*rsqrt(d2)^3.
+
+constexpr int kStack = 8 * kDqLen;
+
+template <bool KAHAN>
+__global__ __launch_bounds__(256) void k_force_bh(const SysDesc* __restrict__ sd, const SysState* __restrict__ st,
+                                                  const BlkWork* __restrict__ work,
+                                                  const float4* __restrict__ node4, const int* __restrict__ child,
+                                                  const int* __restrict__ sorted, float4* __restrict__ acc4) {
+    __shared__ int stk[4][kStack];
+    __shared__ float dqs[kDqLen];
+    const BlkWork w = work[blockIdx.x];
+    const SysDesc d = sd[w.sys];
+    const SysState* S = st + w.sys;
+    if (S->status) return;
+    for (int i = threadIdx.x; i < kDqLen; i += blockDim.x) dqs[i] = S->dq[i];
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int first = w.begin + wave * 64;
+    if (first >= w.end) return;
+    const int idx = first + lane;
+    const bool valid = idx < w.end;
+    const int b = sorted[d.body_off + (valid ? idx : w.end - 1)];
+    const float4* __restrict__ nodes = node4 + d.node_off;
+    const float4 p = nodes[b];
+    const int* __restrict__ C = child + 8 * (size_t)d.cell_off;
+    const int n = d.n;
+    const float e2 = d.eps * d.eps;
+    float ax = 0.f, ay = 0.f, az = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
+    int nint = 0;
+    int* sk = stk[wave];
+    sk[0] = 0;
+    int sp = 1;
+    while (sp > 0) {
+        --sp;
+        const int e = __builtin_amdgcn_readfirstlane(sk[sp]);
+        const int node = e >> 7;
+        const int lvl = e & 127;
+        const float dql = dqs[lvl];
+        const int4* c4 = reinterpret_cast<const int4*>(C + 8 * node);
+        const int4 s0 = c4[0], s1 = c4[1];
+        int vs[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
+        float4 qs[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            vs[k] = __builtin_amdgcn_readfirstlane(vs[k]);
+            qs[k] = nodes[vs[k] < 0 ? 0 : vs[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int v = vs[k];
+            if (v < 0) continue;
+            const float dx = qs[k].x - p.x, dy = qs[k].y - p.y, dz = qs[k].z - p.z;
+            const float d2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, e2)));
+            bool take = v < n;
+            if (!take) take = __all(d2 >= dql);
+            if (take) {
+                const float ri = __builtin_amdgcn_rsqf(d2);
+                const float F = qs[k].w * (ri * ri * ri);
+                if (KAHAN) {
+                    float y, t;
+                    y = dx * F - kx; t = ax + y; kx = (t - ax) - y; ax = t;
+                    y = dy * F - ky; t = ay + y; ky = (t - ay) - y; ay = t;
+                    y = dz * F - kz; t = az + y; kz = (t - az) - y; az = t;
+                } else {
+                    ax = fmaf(dx, F, ax);
+                    ay = fmaf(dy, F, ay);
+                    az = fmaf(dz, F, az);
+                }
+                ++nint;
+            } else {
+                sk[sp] = ((v - n) << 7) | (lvl + 1);
+                ++sp;
+            }
+        }
+    }
+    if (valid) {
+        if (KAHAN) { ax -= kx; ay -= ky; az -= kz; }
+        acc4[d.body_off + b] = make_float4(ax * d.G, ay * d.G, az * d.G, (float)nint);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// brute force: all pairs, one target per lane, j-bodies read wave-uniformly.
+// LAW 0: d*m*rsqrt(|d|^2+eps^2)^3.  LAW 1 (brute_force_update.cl:54-58,114-120):
+// n = normalize(d), n*m/(|n|^2+ueps^2).  Self is skipped.
+
+template <int LAW, bool KAHAN>
+__global__ __launch_bounds__(256) void k_brute(const SysDesc* __restrict__ sd, const SysState* __restrict__ st,
+                                               const BlkWork* __restrict__ work,
+                                               const float4* __restrict__ node4, float4* __restrict__ acc4) {
+    const BlkWork w = work[blockIdx.x];
+    const SysDesc d = sd[w.sys];
+    if (st[w.sys].status) return;
+    const int i = w.begin + threadIdx.x;
+    const bool valid = i < w.end;
+    const int ic = valid ? i : w.end - 1;
+    const float4* __restrict__ nodes = node4 + d.node_off;
+    const float4 p = nodes[ic];
+    const float e = LAW == 0 ? d.eps : d.ueps;
+    const float e2 = e * e;
+    float ax = 0.f, ay = 0.f, az = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
+    const int n = d.n;
+    for (int j = 0; j < n; ++j) {
+        const float4 q = nodes[j];
+        const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+        float fx, fy, fz, s;
+        if (LAW == 0) {
+            const float d2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, e2)));
+            const float ri = __builtin_amdgcn_rsqf(d2);
+            s = q.w * (ri * ri * ri);
+            fx = dx; fy = dy; fz = dz;
+        } else {
+            const float l2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            const float ri = l2 > 0.0f ? __builtin_amdgcn_rsqf(l2) : 0.0f;
+            fx = dx * ri; fy = dy * ri; fz = dz * ri;
+            const float d2 = fmaf(fz, fz, fmaf(fy, fy, fmaf(fx, fx, e2)));
+            s = q.w * __builtin_amdgcn_rcpf(d2);
+        }
+        s = (j == ic) ? 0.0f : s;
+        if (KAHAN) {
+            float y, t;
+            y = fx * s - kx; t = ax + y; kx = (t - ax) - y; ax = t;
+            y = fy * s - ky; t = ay + y; ky = (t - ay) - y; ay = t;
+            y = fz * s - kz; t = az + y; kz = (t - az) - y; az = t;
+        } else {
+            ax = fmaf(fx, s, ax);
+            ay = fmaf(fy, s, ay);
+            az = fmaf(fz, s, az);
+        }
+    }
+    if (valid) {
+        if (KAHAN) { ax -= kx; ay -= ky; az -= kz; }
+        acc4[d.body_off + i] = make_float4(ax * d.G, ay * d.G, az * d.G, (float)(n - 1));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// update (calculate_forces.cl:169-182 EULER; kick-drift-kick for LEAPFROG).
+// mode 0: v += a*dt, x += v*dt.  mode 1: v += a*dt/2.  mode 2: v += a*dt/2, x += v*dt.
+
+__global__ __launch_bounds__(256) void k_update(const SysDesc* __restrict__ sd, const SysState* __restrict__ st,
+                                                const BlkWork* __restrict__ work, float4* node4, float4* vel4,
+                                                const float4* __restrict__ acc4, int mode) {
+    const BlkWork w = work[blockIdx.x];
+    const SysDesc d = sd[w.sys];
+    if (st[w.sys].status) return;
+    const int i = w.begin + threadIdx.x;
+    if (i >= w.end) return;
+    const float4 a = acc4[d.body_off + i];
+    float4 v = vel4[d.body_off + i];
+    float4 x = node4[d.node_off + i];
+    const float h = mode == 0 ? d.dt : 0.5f * d.dt;
+    v.x = fmaf(a.x, h, v.x);
+    v.y = fmaf(a.y, h, v.y);
+    v.z = fmaf(a.z, h, v.z);
+    vel4[d.body_off + i] = v;
+    if (mode != 1) {
+        x.x = fmaf(v.x, d.dt, x.x);
+        x.y = fmaf(v.y, d.dt, x.y);
+        x.z = fmaf(v.z, d.dt, x.z);
+        node4[d.node_off + i] = x;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers
+
+#define NB_LAUNCH(kern, grid, block, stream, ...)                                  \
+    do {                                                                           \
+        if ((grid) > 0) hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, stream, __VA_ARGS__); \
+    } while (0)
+
+void launch_bounds(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
+                   const float4* node4, float* partial) {
+    NB_LAUNCH(k_bounds, nblk, 256, s, sd, st, work, node4, partial);
+}
+void launch_tree_clear(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
+                       int* child, int* cpend) {
+    NB_LAUNCH(k_tree_clear, nblk, 256, s, sd, st, work, child, cpend);
+}
+void launch_tree_init(hipStream_t s, int nsys, SysState* st) {
+    NB_LAUNCH(k_tree_init, (nsys + 255) / 256, 256, s, nsys, st);
+}
+void launch_build(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
+                  const float4* node4, int* child, int* cparent, int* cpend) {
+    NB_LAUNCH(k_build, nblk, 256, s, sd, st, work, node4, child, cparent, cpend);
+}
+void launch_summarize(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
+                      float4* node4, const int* child, const int* cparent, int* cpend, int* ccount, int* cstart) {
+    NB_LAUNCH(k_summarize, nblk, 256, s, sd, st, work, node4, child, cparent, cpend, ccount, cstart);
+}
+void launch_sort(hipStream_t s, int grid, const SysDesc* sd, const SysState* st, int nsys, int total_cap,
+                 const int* child, const int* ccount, int* cstart, int* sorted) {
+    NB_LAUNCH(k_sort, grid, 256, s, sd, st, nsys, total_cap, child, ccount, cstart, sorted);
+}
+void launch_force_bh(hipStream_t s, int nblk, bool kahan, const SysDesc* sd, const SysState* st,
+                     const BlkWork* work, const float4* node4, const int* child, const int* sorted, float4* acc4) {
+    if (kahan)
+        NB_LAUNCH(k_force_bh<true>, nblk, 256, s, sd, st, work, node4, child, sorted, acc4);
+    else
+        NB_LAUNCH(k_force_bh<false>, nblk, 256, s, sd, st, work, node4, child, sorted, acc4);
+}
+void launch_brute(hipStream_t s, int nblk, int law, bool kahan, const SysDesc* sd, const SysState* st,
+                  const BlkWork* work, const float4* node4, float4* acc4) {
+    if (law == 0 && !kahan) NB_LAUNCH((k_brute<0, false>), nblk, 256, s, sd, st, work, node4, acc4);
+    if (law == 0 && kahan) NB_LAUNCH((k_brute<0, true>), nblk, 256, s, sd, st, work, node4, acc4);
+    if (law == 1 && !kahan) NB_LAUNCH((k_brute<1, false>), nblk, 256, s, sd, st, work, node4, acc4);
+    if (law == 1 && kahan) NB_LAUNCH((k_brute<1, true>), nblk, 256, s, sd, st, work, node4, acc4);
+}
+void launch_update(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
+                   float4* node4, float4* vel4, const float4* acc4, int mode) {
+    NB_LAUNCH(k_update, nblk, 256, s, sd, st, work, node4, vel4, acc4, mode);
+}
+
+int sort_blocks_per_cu() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sort, 256, 0) != hipSuccess) return 1;
+    return nb > 0 ? nb : 1;
+}
+
+}  // namespace nb

@@ -1,0 +1,167 @@
+"""numpy-facing wrapper over the C ABI: one NBodySim per nb_sim handle.
+
+Mirrors the stage sequence of CMyApp::Update (src/MyApp.cpp:583-598):
+bounds -> tree_reset -> tree_build -> summarize -> sort -> force -> update, and
+the brute-force branch (src/MyApp.cpp:601-630).  All compute runs in
+libnbody.so on the GPU; this module only moves numpy arrays across the ABI.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _fp(a):
+    return a.ctypes.data_as(L._FP) if a is not None else None
+
+
+def _ip(a):
+    return a.ctypes.data_as(L._IP) if a is not None else None
+
+
+def make_config(n: int, algorithm: int = L.BARNES_HUT, *, method: int = L.EULER, rules: int = L.RULES_REFERENCE,
+                law: int = L.LAW_REFERENCE_UNIT, accumulation: int = L.ACCUM_PLAIN, G: float = 1.0,
+                theta: float = 0.5, eps: float = 0.01, dt: float = 1e-3, cell_capacity: int = 0,
+                use_graph: bool = True, time_stages: bool = False) -> L.NbConfig:
+    lib = L.load()
+    c = L.NbConfig()
+    lib.nb_config_default(c)
+    c.num_bodies, c.algorithm, c.method, c.rules = n, algorithm, method, rules
+    c.law, c.accumulation, c.gravitational_constant = law, accumulation, G
+    c.theta, c.epsilon, c.dt, c.cell_capacity = theta, eps, dt, cell_capacity
+    c.use_graph, c.time_stages = int(use_graph), int(time_stages)
+    return c
+
+
+class NBodySim:
+    def __init__(self, cfg: L.NbConfig, stream: int | None = None):
+        self.lib = L.load()
+        self.cfg = cfg
+        self.n = cfg.num_bodies
+        h = C.c_void_p()
+        L.check(self.lib.nb_sim_create(C.byref(cfg), C.c_void_p(stream) if stream else None, C.byref(h)),
+                "nb_sim_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.nb_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _call(self, name, *args):
+        L.check(getattr(self.lib, name)(self.h, *args), name)
+
+    # state
+    def upload(self, pos, vel, mass):
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1)
+        mass = np.ascontiguousarray(mass, dtype=np.float32).reshape(-1)
+        vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float32).reshape(-1)
+        assert pos.size == 3 * self.n and mass.size == self.n
+        self._call("nb_sim_upload", _fp(pos), _fp(vel), _fp(mass))
+
+    def download(self):
+        p = np.empty(3 * self.n, np.float32)
+        v = np.empty(3 * self.n, np.float32)
+        a = np.empty(3 * self.n, np.float32)
+        if self.n:
+            self._call("nb_sim_download", _fp(p), _fp(v), _fp(a))
+        return p.reshape(-1, 3), v.reshape(-1, 3), a.reshape(-1, 3)
+
+    def status(self) -> int:
+        return self.lib.nb_sim_status(self.h)
+
+    def clear_status(self):
+        self._call("nb_sim_clear_status")
+
+    def sync(self):
+        self._call("nb_sim_sync")
+
+    # stages
+    def bounds(self):
+        box = np.empty(6, np.float32)
+        self._call("nb_stage_bounds", _fp(box))
+        return box
+
+    def tree_reset(self): self._call("nb_stage_tree_reset")
+    def tree_build(self): self._call("nb_stage_tree_build")
+    def summarize(self): self._call("nb_stage_summarize")
+    def sort(self): self._call("nb_stage_sort")
+    def force(self): self._call("nb_stage_force")
+    def brute_force(self): self._call("nb_stage_brute_force")
+    def update(self): self._call("nb_stage_update")
+
+    def force_pipeline(self):
+        """bounds..force on the current positions; returns the box."""
+        box = self.bounds()
+        self.tree_reset()
+        self.tree_build()
+        self.summarize()
+        self.sort()
+        self.force()
+        return box
+
+    def step(self, nsteps: int = 1):
+        self._call("nb_step", int(nsteps))
+
+    # inspection
+    def tree_info(self):
+        info = L.NbTreeInfo()
+        self._call("nb_tree_info_get", C.byref(info))
+        return {"num_cells": info.num_cells, "max_depth": info.max_depth, "capacity": info.capacity,
+                "status": info.status}
+
+    def export_bodies(self, paths: bool = True):
+        s = np.empty(self.n, np.int32)
+        pth = np.empty(self.n * L.PATH_STRIDE, np.uint8) if paths else None
+        ln = np.empty(self.n, np.int32) if paths else None
+        self._call("nb_export_bodies", _ip(s), pth.ctypes.data_as(L._U8P) if paths else None, _ip(ln))
+        return s, (pth.reshape(self.n, L.PATH_STRIDE) if paths else None), ln
+
+    def export_cells(self):
+        nc = self.tree_info()["num_cells"]
+        lv = np.empty(nc, np.int32)
+        sl = np.empty(nc, np.int32)
+        ct = np.empty(nc, np.int32)
+        com = np.empty(4 * nc, np.float32)
+        self._call("nb_export_cells", _ip(lv), _ip(sl), _ip(ct), _fp(com))
+        return lv, sl, ct, com.reshape(-1, 4)
+
+    def export_dq(self):
+        dq = np.empty(L.DQ_LEN, np.float32)
+        self._call("nb_export_dq", _fp(dq))
+        return dq
+
+    def interactions(self) -> int:
+        t = C.c_int64()
+        self._call("nb_force_interactions", C.byref(t))
+        return t.value
+
+    def stage_times(self):
+        ms = np.zeros(L.NUM_STAGES, np.float32)
+        self._call("nb_stage_times", _fp(ms))
+        return dict(zip(L.STAGE_NAMES, ms.tolist()))
+
+
+def device_info():
+    lib = L.load()
+    cu = C.c_int32()
+    name = C.create_string_buffer(64)
+    st = lib.nb_device_info(C.byref(cu), name, 64)
+    if st != L.NB_OK:
+        return None
+    return {"compute_units": cu.value, "arch": name.value.decode()}

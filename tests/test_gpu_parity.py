@@ -1,0 +1,267 @@
+"""GPU parity: the HIP stages (through the C ABI) against the oracle on the same
+seeded inputs.
+
+Bar: bounding box, tree topology (per-body octant paths, cell preorder levels,
+slots and counts), tree-order permutation, cell centre-of-mass/mass and opening
+thresholds are bit-exact; so is every opening decision (per-body interaction
+counts equal).  Accelerations differ only through the hardware reciprocal
+square root and summation order, so they are held to
+|a_gpu - a_oracle| <= ACC_RTOL * sum_k |term_k| per body (ACC_RTOL = 2e-5).
+"""
+import numpy as np
+import pytest
+
+from nbodysimulation_amd import _lib as L
+from nbodysimulation_amd import ic
+from nbodysimulation_amd.engine import NBodySim, make_config
+
+pytestmark = pytest.mark.gpu
+
+ACC_RTOL = 2e-5
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def dist(kind, n, seed):
+    r = np.random.default_rng(seed)
+    if kind == "normal":
+        return ic.normal_cloud(n, seed)
+    if kind == "plummer":
+        return ic.plummer(n, seed)
+    if kind == "clusters":
+        k = max(n // 3, 1)
+        pos = np.concatenate([r.normal(scale=1e-3, size=(k, 3)), r.normal(size=(n - k, 3)) + 5]).astype(np.float32)
+        return pos, np.zeros_like(pos), r.uniform(0.5, 1.5, n).astype(np.float32)
+    if kind == "cube":
+        pos = r.uniform(-1, 1, size=(n, 3)).astype(np.float32)
+        return pos, np.zeros_like(pos), np.full(n, 1.0 / max(n, 1), np.float32)
+    raise ValueError(kind)
+
+
+def gpu_tree(pos, vel, mass, rules, theta=0.5, eps=0.01, G=1.0, cap=0, accum=L.ACCUM_PLAIN):
+    n = len(mass)
+    s = NBodySim(make_config(n, L.BARNES_HUT, rules=rules, theta=theta, eps=eps, G=G, cell_capacity=cap,
+                             accumulation=accum))
+    s.upload(pos, vel, mass)
+    box = s.force_pipeline()
+    return s, box
+
+
+def compare_tree(s, t, n):
+    info = s.tree_info()
+    assert info["status"] == L.NB_OK
+    assert info["num_cells"] == t.ncells
+    assert info["max_depth"] == t.max_depth
+    ex = t.export()
+    srt, paths, plen = s.export_bodies()
+    assert np.array_equal(plen, ex["path_len"])
+    assert np.array_equal(paths, ex["paths"])
+    assert np.array_equal(srt, ex["sorted"])
+    lv, sl, ct, com = s.export_cells()
+    assert np.array_equal(lv, ex["cell_level"])
+    assert np.array_equal(sl, ex["cell_slot"])
+    assert np.array_equal(ct, ex["cell_count"])
+    assert com.tobytes() == ex["cell_com"].tobytes()
+
+
+@pytest.mark.parametrize("rules", [0, 1])
+@pytest.mark.parametrize("kind,n", [("normal", 1), ("normal", 2), ("normal", 63), ("normal", 64), ("normal", 65),
+                                    ("plummer", 1000), ("clusters", 3000), ("cube", 4096), ("plummer", 65536)])
+def test_tree_and_force_parity(orc, gpu, rules, kind, n):
+    say(f"case rules={rules} kind={kind} n={n}")
+    pos, vel, mass = dist(kind, n, 100 + n)
+    theta, eps, G = 0.5, 0.01, 1.0
+    s, box = gpu_tree(pos, vel, mass, rules, theta, eps, G)
+    try:
+        t = orc.Tree(pos, mass, rules, cap=s.tree_info()["capacity"])
+        assert t.status == orc.OK
+        assert box.tobytes() == t.box.tobytes()
+        compare_tree(s, t, n)
+        assert s.export_dq().tobytes() == orc.dq_table(rules, box, theta, eps).tobytes()
+        a_o, absacc, ni = t.force(theta, eps, G, vote=64)
+        _, _, a_g = s.download()
+        assert s.interactions() == int(ni.sum())
+        err = np.linalg.norm(a_g - a_o, axis=1)
+        assert (err <= ACC_RTOL * absacc + 1e-30).all(), float((err / np.maximum(absacc, 1e-30)).max())
+    finally:
+        s.close()
+
+
+def test_kahan_force_parity(orc, gpu):
+    pos, vel, mass = ic.plummer(20000, 9)
+    s, box = gpu_tree(pos, vel, mass, 1, accum=L.ACCUM_KAHAN)
+    try:
+        t = orc.Tree(pos, mass, 1, cap=s.tree_info()["capacity"])
+        a_o, absacc, _ = t.force(0.5, 0.01, 1.0, vote=64, kahan=True)
+        _, _, a_g = s.download()
+        assert (np.linalg.norm(a_g - a_o, axis=1) <= ACC_RTOL * absacc).all()
+    finally:
+        s.close()
+
+
+@pytest.mark.parametrize("algo,law,accum", [(L.BRUTE_FORCE_GLOBAL, L.LAW_REFERENCE_UNIT, 0),
+                                            (L.BRUTE_FORCE_LOCAL, L.LAW_REFERENCE_UNIT, 0),
+                                            (L.BRUTE_FORCE_GLOBAL, L.LAW_PLUMMER, 0),
+                                            (L.BRUTE_FORCE_LOCAL, L.LAW_PLUMMER, 1)])
+@pytest.mark.parametrize("n", [1, 2, 100, 4096, 4097])
+def test_brute_force_parity(orc, gpu, algo, law, accum, n):
+    say(f"case bf algo={algo} law={law} accum={accum} n={n}")
+    pos, vel, mass = ic.normal_cloud(n, n)
+    eps = 0.01
+    ueps = 0.1 if algo == L.BRUTE_FORCE_LOCAL else 0.01  # brute_force_update.cl:56 / :116
+    with NBodySim(make_config(n, algo, law=law, accumulation=accum, eps=eps, G=1.0)) as s:
+        s.upload(pos, vel, mass)
+        s.brute_force()
+        _, _, a_g = s.download()
+    a_o, absacc = orc.force_bf(pos, mass, law, eps if law == L.LAW_PLUMMER else ueps, 1.0, kahan=bool(accum))
+    err = np.linalg.norm(a_g - a_o, axis=1)
+    assert (err <= ACC_RTOL * absacc + 1e-30).all()
+
+
+def test_brute_force_reference_workload_100_steps(orc, gpu):
+    """Plain brute force, N=4096, Normal(0,1), m~U[0.5,1.5]/N, v=0, G=1, dt=1e-3, 100 Euler
+    steps, seed 0, the reference's update-kernel law (unit vectors, eps 0.01)."""
+    n, steps, dt = 4096, 100, 1e-3
+    pos, vel, mass = ic.normal_cloud(n, 0)
+    with NBodySim(make_config(n, L.BRUTE_FORCE_GLOBAL, law=L.LAW_REFERENCE_UNIT, G=1.0, dt=dt)) as s:
+        s.upload(pos, vel, mass)
+        s.step(steps)
+        p_g, v_g, _ = s.download()
+    p, v = pos.copy(), vel.copy()
+    for _ in range(steps):
+        a, _ = orc.force_bf(p, mass, orc.LAW_REFERENCE_UNIT, 0.01, 1.0)
+        v, p = orc.euler(a, v, p, dt)
+    # the unit law is smooth: 100 steps keep the trajectories within 1e-5 of each other
+    assert np.abs(v_g - v).max() <= 1e-5 * np.abs(v).max()
+    assert np.abs(p_g - p).max() <= 1e-5
+
+
+@pytest.mark.parametrize("rules", [0, 1])
+def test_step_matches_oracle_and_is_deterministic(orc, gpu, rules):
+    n = 20000
+    pos, vel, mass = ic.plummer(n, 4)
+    theta, eps, G, dt = 0.5, 0.01, 1.0, 1e-3
+    outs = []
+    for use_graph in (True, False):
+        with NBodySim(make_config(n, L.BARNES_HUT, rules=rules, theta=theta, eps=eps, G=G, dt=dt,
+                                  use_graph=use_graph)) as s:
+            s.upload(pos, vel, mass)
+            s.step(1)
+            outs.append(s.download())
+            s.step(4)
+            outs.append(s.download())
+    for a, b in zip(outs[0], outs[2]):
+        assert a.tobytes() == b.tobytes()  # graph replay == stream launches, bit for bit
+    for a, b in zip(outs[1], outs[3]):
+        assert a.tobytes() == b.tobytes()
+    t = orc.Tree(pos, mass, rules)
+    a_o, absacc, _ = t.force(theta, eps, G, vote=64)
+    v_o, p_o = orc.euler(a_o, vel, pos, dt)
+    p_g, v_g, a_g = outs[0]
+    assert (np.linalg.norm(a_g - a_o, axis=1) <= ACC_RTOL * absacc).all()
+    assert np.abs(v_g - v_o).max() <= ACC_RTOL * absacc.max() * dt * 2
+    assert np.abs(p_g - p_o).max() <= 1e-6 * (1 + np.abs(p_o).max())
+
+
+def test_leapfrog_kdk(orc, gpu):
+    n = 4096
+    pos, vel, mass = ic.plummer(n, 8)
+    theta, eps, G, dt = 0.5, 0.01, 1.0, 1e-3
+    with NBodySim(make_config(n, L.BARNES_HUT, rules=1, method=L.LEAPFROG, theta=theta, eps=eps, G=G,
+                              dt=dt)) as s:
+        s.upload(pos, vel, mass)
+        s.step(1)
+        p_g, v_g, a_g = s.download()
+    t0 = orc.Tree(pos, mass, 1)
+    a0, _, _ = t0.force(theta, eps, G, vote=64)
+    vh = orc.kick(a0, vel, 0.5 * dt)
+    p1 = orc.drift(vh, pos, dt)
+    t1 = orc.Tree(p1, mass, 1)
+    a1, absacc, _ = t1.force(theta, eps, G, vote=64)
+    v1 = orc.kick(a1, vh, 0.5 * dt)
+    assert np.abs(p_g - p1).max() <= 1e-6 * (1 + np.abs(p1).max())
+    assert (np.linalg.norm(a_g - a1, axis=1) <= 1e-4 * absacc + 1e-6).all()
+    assert np.abs(v_g - v1).max() <= 1e-5 * (1 + np.abs(v1).max())
+
+
+# ---------------------------------------------------------------- failure handling (forced from the host)
+
+def test_tree_overflow_reported(gpu):
+    pos, vel, mass = ic.plummer(5000, 1)
+    with NBodySim(make_config(5000, L.BARNES_HUT, cell_capacity=64)) as s:
+        s.upload(pos, vel, mass)
+        with pytest.raises(L.NbError) as e:
+            s.step(1)
+        assert e.value.status == L.NB_E_TREE_OVERFLOW
+        p, v, _ = s.download()
+        assert p.tobytes() == pos.tobytes()  # a failed step leaves the state untouched
+
+
+def test_coincident_bodies_depth_overflow(gpu):
+    pos = np.zeros((3, 3), np.float32)
+    pos[2] = 1.0
+    with NBodySim(make_config(3, L.BARNES_HUT, rules=1)) as s:
+        s.upload(pos, None, np.ones(3, np.float32))
+        with pytest.raises(L.NbError) as e:
+            s.step(1)
+        assert e.value.status == L.NB_E_DEPTH_OVERFLOW
+
+
+def test_nonfinite_reported_and_cleared(gpu):
+    pos, vel, mass = ic.plummer(1000, 2)
+    bad = pos.copy()
+    bad[17, 1] = np.nan
+    with NBodySim(make_config(1000, L.BARNES_HUT)) as s:
+        s.upload(bad, vel, mass)
+        with pytest.raises(L.NbError) as e:
+            s.step(1)
+        assert e.value.status == L.NB_E_NONFINITE
+        s.clear_status()
+        s.upload(pos, vel, mass)
+        s.step(2)
+        assert s.status() == L.NB_OK
+
+
+def test_stage_order_enforced(gpu):
+    pos, vel, mass = ic.plummer(100, 2)
+    with NBodySim(make_config(100, L.BARNES_HUT)) as s:
+        s.upload(pos, vel, mass)
+        with pytest.raises(L.NbError) as e:
+            s.force()
+        assert e.value.status == L.NB_E_STATE
+        s.bounds()
+        with pytest.raises(L.NbError):
+            s.summarize()
+
+
+def test_zero_bodies(gpu):
+    with NBodySim(make_config(0, L.BARNES_HUT)) as s:
+        s.upload(np.zeros((0, 3), np.float32), None, np.zeros(0, np.float32))
+        s.step(3)
+
+
+# ---------------------------------------------------------------- full size (1M Plummer)
+
+@pytest.mark.parametrize("rules", [0, 1])
+def test_full_size_tree_exact(orc, gpu, rules):
+    """N = 1,048,576 Plummer (seed 1): bit-exact tree, sort and box against the oracle,
+    exact interaction counts and accelerations within tolerance on 64 sampled vote groups."""
+    n = 1 << 20
+    say(f"case full-size rules={rules}")
+    pos, vel, mass = ic.plummer(n, 1)
+    s, box = gpu_tree(pos, vel, mass, rules, theta=0.5, eps=0.01, G=1.0)
+    try:
+        assert box.tobytes() == np.concatenate([pos.min(0), pos.max(0)]).astype(np.float32).tobytes()
+        t = orc.Tree(pos, mass, rules, cap=s.tree_info()["capacity"])
+        assert t.status == orc.OK
+        compare_tree(s, t, n)
+        groups = np.random.default_rng(0).choice(n // 64, 64, replace=False).astype(np.int32)
+        a_o, absacc, ni = t.force(0.5, 0.01, 1.0, vote=64, groups=groups)
+        _, _, a_g = s.download()
+        members = t.sorted[(groups[:, None] * 64 + np.arange(64)[None, :]).reshape(-1)]
+        err = np.linalg.norm(a_g[members] - a_o[members], axis=1)
+        assert (err <= ACC_RTOL * absacc[members]).all()
+    finally:
+        s.close()

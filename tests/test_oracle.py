@@ -1,0 +1,383 @@
+"""Pins the CPU oracle (oracle/nbody_oracle.c) against things other than itself:
+hand-computed known answers from the reference's formulas, an independent
+integer (Morton) octree definition, brute-force prefix enumeration, FP64 sums,
+closed forms, symmetry and conservation.  No test here restates an oracle
+formula in Python.
+"""
+import math
+
+import numpy as np
+import pytest
+
+R_REF, R_STD = 0, 1
+
+
+def rng(seed):
+    return np.random.default_rng(seed)
+
+
+# ---------------------------------------------------------------- bounding box
+
+def test_bbox_matches_numpy_bitwise(orc):
+    for seed in range(5):
+        pos = rng(seed).normal(size=(1000 + seed, 3)).astype(np.float32)
+        box, st = orc.bbox(pos)
+        assert st == orc.OK
+        want = np.concatenate([pos.min(0), pos.max(0)]).astype(np.float32)
+        assert box.tobytes() == want.tobytes()
+
+
+def test_bbox_negative_zero_canonical(orc):
+    pos = np.array([[-0.0, 1.0, 2.0], [-0.0, 3.0, 4.0]], np.float32)
+    box, _ = orc.bbox(pos)
+    assert not np.signbit(box[0]) and not np.signbit(box[3])
+
+
+def test_bbox_nonfinite_flagged(orc):
+    pos = np.zeros((4, 3), np.float32)
+    pos[2, 1] = np.nan
+    assert orc.bbox(pos)[1] == orc.E_NONFINITE
+
+
+# ---------------------------------------------------------------- octant rules (hand KATs)
+# Box [-1,1]^3.  calculateOctantIdx (build_tree.hcl:14-29): 4*[y<c] + 2*[x<c] + [z<c].
+
+CUBE = np.array([-1, -1, -1, 1, 1, 1], np.float32)
+
+
+@pytest.mark.parametrize("p,digit", [
+    ((0.5, 0.5, 0.5), 0), ((-0.5, 0.5, 0.5), 2), ((0.5, -0.5, 0.5), 4), ((0.5, 0.5, -0.5), 1),
+    ((-0.5, -0.5, -0.5), 7), ((-0.5, 0.5, -0.5), 3), ((0.5, -0.5, -0.5), 5), ((-0.5, -0.5, 0.5), 6),
+    ((0.0, 0.5, 0.5), 2),  # exact tie: lower half
+])
+def test_first_octant_kat(orc, p, digit):
+    std = orc.path_of(R_STD, np.array(p, np.float32), CUBE, 3)
+    ref = orc.path_of(R_REF, np.array(p, np.float32), CUBE, 3)
+    assert std[0] == digit
+    assert ref[0] == 0 and ref[1] == digit  # reference: root hop into slot 0 (build_tree.cl:57-58)
+
+
+def test_reference_y_halving_follows_x(orc):
+    # (0.75,-0.25,0.75): level-1 octant 4 (y<0).  adjustBoundaryValues picks the y half by
+    # diffVec.x (build_tree.hcl:42): x>0 so y -> [0,1], centre 0.5, y=-0.25 < 0.5 -> bit 4 again.
+    # Consistent halving keeps y in [-1,0], centre -0.5, y > -0.5 -> bit 0.
+    p = np.array([0.75, -0.25, 0.75], np.float32)
+    ref = orc.path_of(R_REF, p, CUBE, 3)
+    std = orc.path_of(R_STD, p, CUBE, 2)
+    assert list(ref[:3]) == [0, 4, 4]
+    assert list(std[:2]) == [4, 0]
+
+
+def morton_digits(p, levels):
+    """Standard octree digits from integer cell coordinates in the root cube [-1,1]^3
+    (a coordinate exactly on a cell boundary belongs to the lower cell)."""
+    out = []
+    scale = 2.0 ** levels
+    ix, iy, iz = (int(math.ceil((c + 1.0) / 2.0 * scale)) - 1 for c in p)
+    for l in range(levels):
+        sh = levels - 1 - l
+        bx, by, bz = (ix >> sh) & 1, (iy >> sh) & 1, (iz >> sh) & 1
+        out.append(4 * (1 - by) + 2 * (1 - bx) + (1 - bz))
+    return out
+
+
+def test_standard_digits_equal_morton(orc):
+    r = rng(1)
+    for _ in range(300):
+        p = r.uniform(-1, 1, 3).astype(np.float32)
+        d = orc.path_of(R_STD, p, CUBE, 18)
+        assert list(d) == morton_digits([float(x) for x in p], 18)
+
+
+def test_standard_point_stays_in_its_box(orc):
+    # containment: STANDARD boxes always contain the body; REFERENCE keeps x and z only
+    r = rng(2)
+    for _ in range(50):
+        p = r.normal(size=3).astype(np.float32)
+        box = np.concatenate([np.minimum(p, -3), np.maximum(p, 3)]).astype(np.float32)
+        d = orc.path_of(R_STD, p, box, 20)
+        lo, hi = orc.root_box(R_STD, box)
+        for dig in d:
+            c = (lo + hi) * np.float32(0.5)
+            bits = np.array([(dig >> 1) & 1, (dig >> 2) & 1, dig & 1])  # x, y, z "lower" bits
+            for k in range(3):
+                if bits[k]:
+                    hi[k] = c[k]
+                else:
+                    lo[k] = c[k]
+            assert np.all(p >= lo) and np.all(p <= hi)
+
+
+# ---------------------------------------------------------------- tree topology vs brute force
+
+def prefix_tree(orc, pos, rules, box):
+    """Independent definition: a cell for every path prefix shared by >= 2 bodies
+    (plus the root); each body's leaf path is its shortest unshared prefix."""
+    n = len(pos)
+    paths = [tuple(orc.path_of(rules, pos[i], box, 65)) for i in range(n)]
+    leaf = []
+    for i in range(n):
+        lcp = 0
+        for j in range(n):
+            if j != i:
+                k = 0
+                while k < 65 and paths[i][k] == paths[j][k]:
+                    k += 1
+                lcp = max(lcp, k)
+        leaf.append(paths[i][:lcp + 1])
+    cells = {(): n}
+    for i in range(n):
+        for L in range(1, len(leaf[i])):
+            cells[leaf[i][:L]] = cells.get(leaf[i][:L], 0) + 1
+    return leaf, cells
+
+
+@pytest.mark.parametrize("rules", [R_REF, R_STD])
+@pytest.mark.parametrize("seed,n,kind", [(0, 2, "normal"), (1, 3, "normal"), (2, 17, "normal"), (3, 64, "normal"),
+                                         (4, 40, "cluster"), (5, 50, "grid")])
+def test_tree_equals_prefix_definition(orc, rules, seed, n, kind):
+    r = rng(seed)
+    if kind == "normal":
+        pos = r.normal(size=(n, 3))
+    elif kind == "cluster":
+        pos = np.concatenate([r.normal(scale=1e-3, size=(n // 2, 3)), r.normal(size=(n - n // 2, 3))])
+    else:
+        g = np.arange(4) - 1.5
+        pos = np.stack(np.meshgrid(g, g, g), -1).reshape(-1, 3)[:n] + r.normal(scale=1e-3, size=(n, 3))
+    pos = pos.astype(np.float32)
+    mass = r.uniform(0.5, 1.5, n).astype(np.float32)
+    t = orc.Tree(pos, mass, rules)
+    assert t.status == orc.OK
+    ex = t.export()
+    leaf, cells = prefix_tree(orc, pos, rules, t.box)
+    for i in range(n):
+        L = ex["path_len"][i]
+        assert tuple(ex["paths"][i][:L]) == leaf[i]
+    assert t.ncells == len(cells)
+    # preorder cells: rebuild prefixes from (level, slot)
+    stack = []
+    for lv, sl, ct in zip(ex["cell_level"], ex["cell_slot"], ex["cell_count"]):
+        stack = stack[:max(lv - 1, 0)]
+        if lv > 0:
+            stack.append(int(sl))
+        assert cells[tuple(stack)] == ct
+    assert t.max_depth == max(1, max(len(x) for x in leaf) - 1)
+
+
+def test_reference_root_has_single_child(orc):
+    pos = rng(7).normal(size=(100, 3)).astype(np.float32)
+    t = orc.Tree(pos, np.ones(100, np.float32), R_REF)
+    ex = t.export()
+    assert (ex["paths"][:, 0] == 0).all()  # build_tree.cl:57-58: every body enters slot 0 of the root
+    assert ex["cell_level"][1] == 1 and ex["cell_slot"][1] == 0 and ex["cell_count"][1] == 100
+
+
+def test_single_and_empty(orc):
+    t = orc.Tree(np.zeros((1, 3), np.float32), np.ones(1, np.float32), R_STD)
+    ex = t.export()
+    assert t.ncells == 1 and ex["path_len"][0] == 1 and list(ex["sorted"]) == [0]
+
+
+def test_overflow_and_depth(orc):
+    pos = rng(3).normal(size=(200, 3)).astype(np.float32)
+    t = orc.Tree(pos, np.ones(200, np.float32), R_STD, cap=20)
+    assert t.status == orc.E_TREE_OVERFLOW
+    pos2 = np.zeros((2, 3), np.float32)  # coincident bodies never separate
+    t2 = orc.Tree(pos2, np.ones(2, np.float32), R_STD, cap=1000)
+    assert t2.status == orc.E_DEPTH_OVERFLOW
+
+
+# ---------------------------------------------------------------- summarize and sort
+
+@pytest.mark.parametrize("rules", [R_REF, R_STD])
+def test_summarize_against_fp64(orc, rules):
+    r = rng(11)
+    n = 3000
+    pos = r.normal(size=(n, 3)).astype(np.float32)
+    mass = r.uniform(0.5, 1.5, n).astype(np.float32)
+    t = orc.Tree(pos, mass, rules)
+    ex = t.export()
+    assert ex["cell_count"][0] == n
+    root = ex["cell_com"][0]
+    assert abs(root[3] - mass.astype(np.float64).sum()) <= n * 1.2e-7 * mass.sum()
+    com64 = (pos.astype(np.float64) * mass[:, None]).sum(0) / mass.astype(np.float64).sum()
+    assert np.allclose(root[:3], com64, rtol=0, atol=1e-5)
+    # every cell: count and COM equal the FP64 values of the bodies below it (by path prefix)
+    pl = ex["paths"]
+    ln = ex["path_len"]
+    stack = []
+    for ci, (lv, sl) in enumerate(zip(ex["cell_level"], ex["cell_slot"])):
+        stack = stack[:max(lv - 1, 0)]
+        if lv > 0:
+            stack.append(int(sl))
+        if ci % 97 and ci > 5:
+            continue
+        pre = np.array(stack, np.uint8)
+        member = (ln > lv) & np.all(pl[:, :lv] == pre, axis=1) if lv else np.ones(n, bool)
+        assert member.sum() == ex["cell_count"][ci]
+        m = mass[member].astype(np.float64)
+        c = (pos[member].astype(np.float64) * m[:, None]).sum(0) / m.sum()
+        assert np.allclose(ex["cell_com"][ci][:3], c, rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize("rules", [R_REF, R_STD])
+def test_sort_is_lexicographic_path_order(orc, rules):
+    r = rng(12)
+    n = 2000
+    pos = r.standard_cauchy(size=(n, 3)).clip(-50, 50).astype(np.float32)
+    t = orc.Tree(pos, np.ones(n, np.float32), rules)
+    ex = t.export()
+    s = ex["sorted"]
+    assert sorted(s.tolist()) == list(range(n))
+    keys = [bytes(ex["paths"][i][:ex["path_len"][i]]) for i in range(n)]
+    assert [keys[i] for i in s] == sorted(keys)
+
+
+# ---------------------------------------------------------------- opening thresholds (KATs)
+
+def test_dq_reference_kat(orc):
+    # calculate_forces.cl:45-62 with box [-1,1]^3: r = 1, dq0 = r*r/theta, dq_l = dq0/4^l, each + eps
+    dq = orc.dq_table(R_REF, CUBE, 0.5, 0.01)
+    assert dq[0] == np.float32(2.0) + np.float32(0.01)
+    assert dq[1] == np.float32(0.5) + np.float32(0.01)
+    assert dq[3] == np.float32(2.0 / 64) + np.float32(0.01)
+    dq0 = orc.dq_table(R_REF, CUBE, 0.0, 0.01)  # theta <= 0 -> r*r
+    assert dq0[0] == np.float32(1.0) + np.float32(0.01)
+
+
+def test_dq_standard_kat(orc):
+    dq = orc.dq_table(R_STD, CUBE, 0.5, 0.01)  # children of the root have edge 1: (1/0.5)^2 + eps^2
+    assert dq[0] == np.float32(4.0) + np.float32(0.01) * np.float32(0.01)
+    assert dq[1] == np.float32(1.0) + np.float32(0.01) * np.float32(0.01)
+    assert np.isinf(orc.dq_table(R_STD, CUBE, 0.0, 0.01)).all()
+
+
+# ---------------------------------------------------------------- forces
+
+def test_two_body_closed_form(orc):
+    pos = np.array([[0, 0, 0], [1.5, 0, 0]], np.float32)
+    mass = np.array([2.0, 3.0], np.float32)
+    eps, G = 0.1, 0.7
+    want0 = G * 3.0 * 1.5 / (1.5 ** 2 + eps ** 2) ** 1.5
+    want1 = -G * 2.0 * 1.5 / (1.5 ** 2 + eps ** 2) ** 1.5
+    for rules in (R_REF, R_STD):
+        t = orc.Tree(pos, mass, rules)
+        a, _, ni = t.force(0.5, eps, G, vote=1)
+        assert np.allclose(a[:, 0], [want0, want1], rtol=2e-6) and np.allclose(a[:, 1:], 0)
+        assert (ni == 2).all()  # self (zero) + partner
+    a, _ = orc.force_bf(pos, mass, orc.LAW_PLUMMER, eps, G)
+    assert np.allclose(a[:, 0], [want0, want1], rtol=2e-6)
+
+
+def test_reference_unit_law_closed_form(orc):
+    # brute_force_update.cl:54-58: n = normalize(d); a = G*m*n/(|n|^2 + e^2); distance-independent
+    pos = np.array([[0, 0, 0], [3.0, 4.0, 0], [0, 0, -0.2]], np.float32)
+    mass = np.array([1.0, 2.0, 5.0], np.float32)
+    G, e = 1.3, 0.01
+    a, _ = orc.force_bf(pos, mass, orc.LAW_REFERENCE_UNIT, e, G)
+    s = G / (1 + e * e)
+    want0 = s * (2.0 * np.array([0.6, 0.8, 0]) + 5.0 * np.array([0, 0, -1.0]))
+    assert np.allclose(a[0], want0, rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("law", [0, 1])
+def test_bf_momentum_conservation(orc, law):
+    r = rng(21)
+    n = 500
+    pos = r.normal(size=(n, 3)).astype(np.float32)
+    mass = r.uniform(0.5, 1.5, n).astype(np.float32)
+    a, absacc = orc.force_bf(pos, mass, law, 0.05, 1.0)
+    p = (a.astype(np.float64) * mass[:, None]).sum(0)
+    assert np.abs(p).max() <= 1e-5 * (absacc * mass).sum()
+
+
+def test_bf_plummer_vs_fp64(orc):
+    r = rng(22)
+    n = 2000
+    pos = r.normal(size=(n, 3)).astype(np.float32)
+    mass = r.uniform(0.5, 1.5, n).astype(np.float32) / n
+    a, absacc = orc.force_bf(pos, mass, orc.LAW_PLUMMER, 0.01, 1.0)
+    a64 = orc.direct_f64(pos, mass, orc.LAW_PLUMMER, 0.01, 1.0)
+    err = np.linalg.norm(a - a64, axis=1)
+    assert (err <= 1e-5 * absacc).all()
+    ak, _ = orc.force_bf(pos, mass, orc.LAW_PLUMMER, 0.01, 1.0, kahan=True)
+    errk = np.linalg.norm(ak - a64, axis=1)
+    assert np.median(errk) <= np.median(err)
+
+
+@pytest.mark.parametrize("rules,theta", [(R_STD, 0.0), (R_REF, 1e-7)])
+def test_bh_theta_zero_is_direct_sum(orc, rules, theta):
+    r = rng(23)
+    n = 1500
+    pos = r.normal(size=(n, 3)).astype(np.float32)
+    mass = r.uniform(0.5, 1.5, n).astype(np.float32) / n
+    t = orc.Tree(pos, mass, rules)
+    a, absacc, ni = t.force(theta, 0.01, 1.0, vote=64)
+    assert (ni == n).all()  # every body, no cell accepted
+    a64 = orc.direct_f64(pos, mass, orc.LAW_PLUMMER, 0.01, 1.0)
+    assert (np.linalg.norm(a - a64, axis=1) <= 1e-5 * absacc).all()
+
+
+def test_bh_standard_accuracy_and_vote(orc):
+    r = rng(24)
+    n = 20000
+    pos = r.normal(size=(n, 3)).astype(np.float32)
+    mass = np.full(n, 1.0 / n, np.float32)
+    t = orc.Tree(pos, mass, R_STD)
+    tg = np.arange(0, n, 37)
+    a64 = orc.direct_f64(pos, mass, orc.LAW_PLUMMER, 0.01, 1.0, targets=tg)
+    a1, _, n1 = t.force(0.5, 0.01, 1.0, vote=1)
+    a64g, _, n64 = t.force(0.5, 0.01, 1.0, vote=64)
+    rel1 = np.linalg.norm(a1[tg] - a64, axis=1) / np.linalg.norm(a64, axis=1)
+    rel64 = np.linalg.norm(a64g[tg] - a64, axis=1) / np.linalg.norm(a64, axis=1)
+    assert np.median(rel1) < 2e-3
+    assert np.median(rel64) <= np.median(rel1)  # a group opens at least what each member opens
+    assert (n64 >= n1).all()
+
+
+def test_cube_symmetry(orc):
+    c = np.array([[x, y, z] for x in (-1, 1) for y in (-1, 1) for z in (-1, 1)], np.float32)
+    pos = np.concatenate([c, np.zeros((1, 3), np.float32)])
+    mass = np.ones(9, np.float32)
+    t = orc.Tree(pos, mass, R_STD)
+    a, _, _ = t.force(0.0, 0.01, 1.0, vote=1)
+    assert np.abs(a[8]).max() < 1e-6
+    mags = np.linalg.norm(a[:8], axis=1)
+    assert np.allclose(mags, mags[0], rtol=1e-6)
+    cos = (a[:8] * -c).sum(1) / (mags * np.linalg.norm(c, axis=1))
+    assert np.allclose(cos, 1.0, atol=1e-6)
+
+
+def test_far_field_monopole(orc):
+    # a tight cluster far away acts as one mass (shell theorem / monopole)
+    r = rng(25)
+    cl = r.normal(scale=0.01, size=(200, 3)) + np.array([100.0, 0, 0])
+    pos = np.concatenate([np.zeros((1, 3)), cl]).astype(np.float32)
+    mass = np.concatenate([[1.0], np.full(200, 0.005)]).astype(np.float32)
+    t = orc.Tree(pos, mass, R_STD)
+    a, _, ni = t.force(0.5, 1e-3, 1.0, vote=1)
+    assert ni[0] < 10  # accepted as a few cells, not 200 bodies
+    assert np.isclose(a[0, 0], 1.0 / 100.0 ** 2, rtol=1e-3)
+
+
+def test_euler_one_step_closed_form(orc):
+    a = np.array([[1.0, -2.0, 0.5]], np.float32)
+    v = np.array([[0.1, 0.2, 0.3]], np.float32)
+    x = np.array([[1.0, 1.0, 1.0]], np.float32)
+    dt = 0.01
+    v1, x1 = orc.euler(a, v, x, dt)
+    v64 = v.astype(np.float64) + a * dt
+    x64 = x.astype(np.float64) + v1.astype(np.float64) * dt
+    assert np.allclose(v1, v64, rtol=1e-7) and np.allclose(x1, x64, rtol=1e-7)
+
+
+def test_bh_step_runs_and_conserves_momentum(orc):
+    r = rng(26)
+    n = 3000
+    pos = r.normal(size=(n, 3)).astype(np.float32)
+    mass = np.full(n, 1.0 / n, np.float32)
+    vel = np.zeros((n, 3), np.float32)
+    st, p1, v1, a1 = orc.bh_steps(pos, vel, mass, R_STD, 3, 0.3, 0.01, 1.0, 1e-3)
+    assert st == orc.OK
+    mom = (v1.astype(np.float64) * mass[:, None]).sum(0)
+    assert np.abs(mom).max() < 1e-3 * np.abs(v1).mean()
