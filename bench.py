@@ -1,0 +1,170 @@
+#!/usr/bin/env python3
+"""Barnes-Hut throughput on one MI355X: timesteps/s at N = 1,048,576 Plummer bodies.
+
+Workload (one "step" = one full pipeline pass): Plummer sphere (M = 1, a = 1,
+equal masses, virial velocities, seed 1), theta = 0.5, eps = 0.01, G = 1,
+dt = 1e-3, Euler; bounds -> reset -> build -> summarize -> sort -> force ->
+update, all on the GPU with the state resident in HBM.
+
+Prints one JSON line.  Besides the throughput it reports:
+  * roofline of the dominant kernel (the force traversal): algorithmic FP32
+    flops = 20 x I_alg per step, where I_alg is the number of interactions a
+    per-body traversal under the same rules needs (estimated by the oracle on
+    a sample of bodies); the kernel time comes from HIP events recorded around
+    every force launch of the timed region on the simulation's stream;
+  * cpu_baseline: the oracle (C, OpenMP) timed on this host on a bounded
+    sample of the same step (full tree build, per-body traversal of a sample
+    of bodies scaled to N);
+  * the force error against an FP64 direct sum on 1024 seeded targets.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "BH timesteps/s @1M bodies θ=0.5; GInteractions/s; ensemble p50/p99 latency"
+PEAK_FP32_TFLOPS = 157.3
+FLOPS_PER_INTERACTION = 20
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--rules", choices=["standard", "reference"], default="standard")
+    ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--cpu-sample", type=int, default=65536)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-error", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each box has one MI355X); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from nbodysimulation_amd import _lib as L
+    from nbodysimulation_amd import device_info, ic
+    from nbodysimulation_amd.engine import NBodySim, make_config
+
+    info = device_info()
+    if info is None:
+        raise SystemExit("no HIP device")
+    rules = L.RULES_STANDARD if args.rules == "standard" else L.RULES_REFERENCE
+    n, theta, eps, G, dt = args.n, args.theta, 0.01, 1.0, 1e-3
+    log(f"bench: device {info}, N={n}, rules={args.rules}, theta={theta}")
+    pos, vel, mass = ic.plummer(n, 1)
+
+    sim = NBodySim(make_config(n, L.BARNES_HUT, rules=rules, theta=theta, eps=eps, G=G, dt=dt,
+                               time_stages=True))
+    sim.upload(pos, vel, mass)
+    if args.warmup:
+        sim.step(args.warmup)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step(args.steps)  # returns after a stream synchronise
+    t1 = time.perf_counter()
+    stages = sim.stage_times()  # ms summed over the timed steps (HIP events)
+    elapsed = t1 - t0
+    steps_per_s = args.steps / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+    exec_inter = sim.interactions()  # last step, executed (wave vote included)
+    tinfo = sim.tree_info()
+    log(f"bench: {steps_per_s:.2f} steps/s, {ms_per_step:.3f} ms/step, stages(ms/step) "
+        + ", ".join(f"{k}={v / args.steps:.3f}" for k, v in stages.items() if v))
+
+    # state at the end of the timed region, and one more force pass on it
+    sim.force_pipeline()
+    p_end, _, a_end = sim.download()
+    sim.close()
+
+    from oracle import oracle as orc
+
+    out_err = None
+    if not args.no_error:
+        tg = np.random.default_rng(12345).choice(n, 1024, replace=False).astype(np.int32)
+        a64 = orc.direct_f64(p_end, mass, orc.LAW_PLUMMER, eps, G, targets=tg)
+        rel = np.linalg.norm(a_end[tg] - a64, axis=1) / np.linalg.norm(a64, axis=1)
+        out_err = {"median": float(np.median(rel)), "p99": float(np.quantile(rel, 0.99)), "targets": 1024,
+                   "against": "FP64 direct sum"}
+        log(f"bench: force rel err vs FP64 median {out_err['median']:.3e} p99 {out_err['p99']:.3e}")
+
+    # oracle on the same state: per-body interaction count (I_alg) and the CPU baseline
+    cpu = None
+    t_a = time.perf_counter()
+    box, _ = orc.bbox(p_end)
+    t_b = time.perf_counter()
+    tree = orc.Tree(p_end, mass, rules, cap=tinfo["capacity"], box=box)
+    t_c = time.perf_counter()
+    sample = min(args.cpu_sample, n)
+    groups = np.sort(np.random.default_rng(7).choice(n, sample, replace=False)).astype(np.int32)
+    _, _, ni = tree.force(theta, eps, G, vote=1, groups=groups)
+    t_d = time.perf_counter()
+    members = tree.sorted[groups]
+    i_alg = float(ni[members].mean()) * n
+    acc_dummy = np.zeros((n, 3), np.float32)
+    t_e = time.perf_counter()
+    orc.euler(acc_dummy, np.zeros((n, 3), np.float32), p_end, dt)
+    t_f = time.perf_counter()
+    cpu_step_s = (t_b - t_a) + (t_c - t_b) + (t_d - t_c) * n / sample + (t_f - t_e)
+    if not args.no_cpu_baseline:
+        cpu = {"value": 1.0 / cpu_step_s, "unit": "timesteps/s", "cores": orc.threads(), "kind": "port",
+               "sample": (f"one step of the same workload on the oracle: bounds, sequential tree build, summarize "
+                          f"and sort over all {n} bodies, per-body traversal of {sample} sampled bodies scaled "
+                          f"to {n}, Euler update; {cpu_step_s:.2f} s per step")}
+        log(f"bench: cpu baseline {cpu['value']:.4f} steps/s ({cpu_step_s:.2f} s/step, {cpu['cores']} threads)")
+
+    force_s = stages["force"] * 1e-3 / args.steps
+    achieved = FLOPS_PER_INTERACTION * i_alg / force_s / 1e12
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_force_bh_plummer_1M.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get(args.rules, {}).get("bytes_per_launch")
+    line = {
+        "metric": METRIC,
+        "value": steps_per_s,
+        "unit": "timesteps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded Plummer sphere (AHW 1974), generated by the library",
+        "config": {"workload": f"barnes-hut Plummer N={n} theta={theta} eps={eps} G={G} dt={dt} Euler seed 1",
+                   "rules": args.rules, "vote_group": 64, "cell_capacity": tinfo["capacity"],
+                   "device": info},
+        "ginteractions_per_s": i_alg * steps_per_s / 1e9,
+        "ginteractions_executed_per_s": exec_inter * steps_per_s / 1e9,
+        "interactions_per_step": {"algorithmic_per_body_traversal": i_alg, "executed_wave_vote": exec_inter},
+        "stage_ms_per_step": {k: v / args.steps for k, v in stages.items()},
+        "force_error": out_err,
+        "roofline": {"bound": "mfma", "note": "FP32 vector ALU (no MFMA); peak = MI355X FP32 157.3 TFLOP/s",
+                     "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                     "kernel": "k_force_bh", "kernel_ms": force_s * 1e3,
+                     "flops_per_launch": FLOPS_PER_INTERACTION * i_alg},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

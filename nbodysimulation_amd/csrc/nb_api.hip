@@ -247,6 +247,7 @@ nb_status nb_sim_sync(nb_sim* sim) {
 }
 
 static nb_status status_of_bits(int bits) {
+    if (bits & nb::kStStall) return fail(NB_E_HIP, "a bounded device wait ran out (inconsistent tree)");
     if (bits & nb::kStNonfinite) return fail(NB_E_NONFINITE, "non-finite body position");
     if (bits & nb::kStDepthOverflow) return fail(NB_E_DEPTH_OVERFLOW, "tree deeper than NB_MAX_LEVEL");
     if (bits & nb::kStTreeOverflow) return fail(NB_E_TREE_OVERFLOW, "cell pool exhausted");

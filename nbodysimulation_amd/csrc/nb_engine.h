@@ -21,7 +21,7 @@ void launch_build(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, cons
                   const float4* node4, int* child, int* cparent, int* cpend);
 void launch_summarize(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
                       float4* node4, const int* child, const int* cparent, int* cpend, int* ccount, int* cstart);
-void launch_sort(hipStream_t s, int grid, const SysDesc* sd, const SysState* st, int nsys, int total_cap,
+void launch_sort(hipStream_t s, int grid, const SysDesc* sd, SysState* st, int nsys, int total_cap,
                  const int* child, const int* ccount, int* cstart, int* sorted);
 void launch_force_bh(hipStream_t s, int nblk, bool kahan, const SysDesc* sd, const SysState* st,
                      const BlkWork* work, const float4* node4, const int* child, const int* sorted, float4* acc4);

@@ -153,6 +153,7 @@ nb_status Engine::init(const std::vector<SysSpec>& systems, void* user_stream) {
     NB_HIP(hipMemsetAsync(acc4, 0, sizeof(float4) * nbd, stream));
     NB_HIP(hipMemsetAsync(child, 0xff, sizeof(int) * 8 * nc, stream));
     NB_HIP(hipMemsetAsync(cpend, 0, sizeof(int) * nc, stream));
+    NB_HIP(hipMemsetAsync(sorted, 0, sizeof(int) * nbd, stream));
     NB_HIP(hipMemsetAsync(ccount, 0, sizeof(int) * nc, stream));
     NB_HIP(hipMemsetAsync(cstart, 0xff, sizeof(int) * nc, stream));
     NB_HIP(hipMemsetAsync(d_state, 0, sizeof(SysState) * std::max(nsys, 1), stream));
@@ -168,7 +169,8 @@ nb_status Engine::init(const std::vector<SysSpec>& systems, void* user_stream) {
     NB_HIP(up(t_brute, tf));
     NB_HIP(up(t_update, tu));
     launch_tree_init(stream, nsys, d_state);
-    sort_grid = sort_blocks_per_cu() * cus;
+    // persistent grid for the top-down sort: half of what fits, so every block is resident
+    sort_grid = std::max(1, sort_blocks_per_cu() * cus / 2);
     NB_HIP(hipGetLastError());
     NB_HIP(hipStreamSynchronize(stream));
     return NB_OK;

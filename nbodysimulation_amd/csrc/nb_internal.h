@@ -28,6 +28,7 @@ constexpr int kLocked = -2;
 constexpr int kStTreeOverflow = 1;
 constexpr int kStDepthOverflow = 2;
 constexpr int kStNonfinite = 4;
+constexpr int kStStall = 8;  // a bounded wait ran out (inconsistent tree); never expected
 
 struct SysDesc {
     int n;          // bodies

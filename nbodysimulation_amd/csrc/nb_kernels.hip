@@ -260,7 +260,12 @@ __global__ __launch_bounds__(256) void k_build(const SysDesc* __restrict__ sd, S
     int node = 0, lvl = 0;
     int dg = level_digit(d.rules, 0, px, py, pz, lx, ly, lz, hx, hy, hz);
     int depth = 1;
+    int tries = 0;
     while (active) {
+        if (++tries > (1 << 22)) {  // cannot happen with a consistent tree; never spin forever
+            atomicOr(&S->status, kStStall);
+            break;
+        }
         int slot = 8 * node + dg;
         int v = ld_c(C + slot);
         while (v >= n) {
@@ -310,7 +315,7 @@ __global__ __launch_bounds__(256) void k_build(const SysDesc* __restrict__ sd, S
                         const int db2 = level_digit(d.rules, l2, px, py, pz, ax, ay, az, bx, by, bz);
                         const int dq2 = level_digit(d.rules, l2, q4.x, q4.y, q4.z, qx0, qy0, qz0, qx1, qy1, qz1);
                         par[c] = parent;
-                        pend[c] = (i < k - 1) ? 1 : 0;
+                        st_c(pend + c, (i < k - 1) ? 1 : 0);
                         if (i < k - 1) {
                             st_c(C + 8 * c + db2, n + c + 1);
                         } else {
@@ -412,7 +417,7 @@ __global__ __launch_bounds__(256) void k_summarize(const SysDesc* __restrict__ s
 // its parent, which always has a smaller number.  The grid is sized to be fully
 // resident, so the lowest unfinished cell can always proceed.
 
-__global__ __launch_bounds__(256) void k_sort(const SysDesc* __restrict__ sd, const SysState* __restrict__ st,
+__global__ __launch_bounds__(256) void k_sort(const SysDesc* __restrict__ sd, SysState* st,
                                               int nsys, int total_cap, const int* __restrict__ child,
                                               const int* __restrict__ ccount, int* cstart, int* sorted) {
     const int T = gridDim.x * blockDim.x;
@@ -421,6 +426,7 @@ __global__ __launch_bounds__(256) void k_sort(const SysDesc* __restrict__ sd, co
     SysDesc d = sd[0];
     int used = min(st[0].ncells, d.cap);
     int bad = st[0].status;
+    int waited = 0;
     while (g < total_cap) {
         while (g >= d.cell_off + d.cap && s + 1 < nsys) {
             ++s;
@@ -435,9 +441,14 @@ __global__ __launch_bounds__(256) void k_sort(const SysDesc* __restrict__ sd, co
         }
         int start = ci == 0 ? 0 : ld_c(cstart + g);
         if (start < 0) {
+            if (++waited > (1 << 24)) {  // parent never published: inconsistent tree
+                atomicOr(&st[s].status, kStStall);
+                return;
+            }
             __builtin_amdgcn_s_sleep(2);
             continue;
         }
+        waited = 0;
         const int4* c4 = reinterpret_cast<const int4*>(child + 8 * (size_t)g);
         int4 a = c4[0], bq = c4[1];
         int vs[8] = {a.x, a.y, a.z, a.w, bq.x, bq.y, bq.z, bq.w};
@@ -446,7 +457,7 @@ __global__ __launch_bounds__(256) void k_sort(const SysDesc* __restrict__ sd, co
             const int v = vs[k];
             if (v < 0) continue;
             if (v < d.n) {
-                sorted[d.body_off + start] = v;
+                if (start < d.n) sorted[d.body_off + start] = v;
                 start += 1;
             } else {
                 const int cg = d.cell_off + (v - d.n);
@@ -486,7 +497,7 @@ __global__ __launch_bounds__(256) void k_force_bh(const SysDesc* __restrict__ sd
     if (first >= w.end) return;
     const int idx = first + lane;
     const bool valid = idx < w.end;
-    const int b = sorted[d.body_off + (valid ? idx : w.end - 1)];
+    const int b = min(max(sorted[d.body_off + (valid ? idx : w.end - 1)], 0), d.n - 1);
     const float4* __restrict__ nodes = node4 + d.node_off;
     const float4 p = nodes[b];
     const int* __restrict__ C = child + 8 * (size_t)d.cell_off;
@@ -501,7 +512,7 @@ __global__ __launch_bounds__(256) void k_force_bh(const SysDesc* __restrict__ sd
         --sp;
         const int e = __builtin_amdgcn_readfirstlane(sk[sp]);
         const int node = e >> 7;
-        const int lvl = e & 127;
+        const int lvl = min(e & 127, kMaxLevel);
         const float dql = dqs[lvl];
         const int4* c4 = reinterpret_cast<const int4*>(C + 8 * node);
         const int4 s0 = c4[0], s1 = c4[1];
@@ -534,7 +545,7 @@ __global__ __launch_bounds__(256) void k_force_bh(const SysDesc* __restrict__ sd
                     az = fmaf(dz, F, az);
                 }
                 ++nint;
-            } else {
+            } else if (sp < kStack) {  // depth <= 64 keeps sp below 7*64+8
                 sk[sp] = ((v - n) << 7) | (lvl + 1);
                 ++sp;
             }
@@ -656,7 +667,7 @@ void launch_summarize(hipStream_t s, int nblk, const SysDesc* sd, const SysState
                       float4* node4, const int* child, const int* cparent, int* cpend, int* ccount, int* cstart) {
     NB_LAUNCH(k_summarize, nblk, 256, s, sd, st, work, node4, child, cparent, cpend, ccount, cstart);
 }
-void launch_sort(hipStream_t s, int grid, const SysDesc* sd, const SysState* st, int nsys, int total_cap,
+void launch_sort(hipStream_t s, int grid, const SysDesc* sd, SysState* st, int nsys, int total_cap,
                  const int* child, const int* ccount, int* cstart, int* sorted) {
     NB_LAUNCH(k_sort, grid, 256, s, sd, st, nsys, total_cap, child, ccount, cstart, sorted);
 }
