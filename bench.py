@@ -50,6 +50,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first run a plain device copy of this many MiB (scale for HBM byte counters)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -64,6 +66,17 @@ def main():
     info = device_info()
     if info is None:
         raise SystemExit("no HIP device")
+    if args.calibrate_copy_mib:
+        import torch  # plumbing only: a copy kernel of known size for counter calibration
+        nel = args.calibrate_copy_mib * (1 << 20) // 4
+        src = torch.ones(nel, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"bench: copy calibration {args.calibrate_copy_mib} MiB x3 (read {args.calibrate_copy_mib} MiB, "
+            f"write {args.calibrate_copy_mib} MiB per launch)")
+        del src, dst
     rules = L.RULES_STANDARD if args.rules == "standard" else L.RULES_REFERENCE
     n, theta, eps, G, dt = args.n, args.theta, 0.01, 1.0, 1e-3
     log(f"bench: device {info}, N={n}, rules={args.rules}, theta={theta}")
@@ -150,6 +163,7 @@ def main():
         "data": "synthetic: seeded Plummer sphere (AHW 1974), generated by the library",
         "config": {"workload": f"barnes-hut Plummer N={n} theta={theta} eps={eps} G={G} dt={dt} Euler seed 1",
                    "rules": args.rules, "vote_group": 64, "cell_capacity": tinfo["capacity"],
+                   "cells_used": tinfo["num_cells"], "max_depth": tinfo["max_depth"],
                    "device": info},
         "ginteractions_per_s": i_alg * steps_per_s / 1e9,
         "ginteractions_executed_per_s": exec_inter * steps_per_s / 1e9,

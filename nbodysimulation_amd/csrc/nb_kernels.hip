@@ -164,19 +164,38 @@ __global__ __launch_bounds__(256) void k_bounds(const SysDesc* __restrict__ sd, 
             for (int k = 3; k < 7; ++k) r[k] = fmaxf(r[k], red[q][k]);
         }
         for (int k = 0; k < 7; ++k) st_cf(partial + 7 * (size_t)blockIdx.x + k, r[k]);
-        int t = __hip_atomic_fetch_add(&S->ticket, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_waitcnt(0);
+        const int t = __hip_atomic_fetch_add(&S->ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         is_last = (t == d.bbox_blocks - 1);
     }
     __syncthreads();
-    if (!is_last || threadIdx.x != 0) return;
-    float r[7];
-    for (int k = 0; k < 3; ++k) r[k] = INFINITY;
-    for (int k = 3; k < 6; ++k) r[k] = -INFINITY;
-    r[6] = 0.0f;
-    for (int q = 0; q < d.bbox_blocks; ++q) {
+    if (!is_last) return;
+    // last workgroup of the system: reduce the partials (coherent loads)
+    lo0 = lo1 = lo2 = INFINITY;
+    hi0 = hi1 = hi2 = -INFINITY;
+    bad = 0.0f;
+    for (int q = threadIdx.x; q < d.bbox_blocks; q += blockDim.x) {
         const float* pp = partial + 7 * (size_t)(d.bbox_first + q);
-        for (int k = 0; k < 3; ++k) r[k] = fminf(r[k], ld_cf(pp + k));
-        for (int k = 3; k < 7; ++k) r[k] = fmaxf(r[k], ld_cf(pp + k));
+        lo0 = fminf(lo0, ld_cf(pp)); lo1 = fminf(lo1, ld_cf(pp + 1)); lo2 = fminf(lo2, ld_cf(pp + 2));
+        hi0 = fmaxf(hi0, ld_cf(pp + 3)); hi1 = fmaxf(hi1, ld_cf(pp + 4)); hi2 = fmaxf(hi2, ld_cf(pp + 5));
+        bad = fmaxf(bad, ld_cf(pp + 6));
+    }
+    lo0 = wave_min(lo0); lo1 = wave_min(lo1); lo2 = wave_min(lo2);
+    hi0 = wave_max(hi0); hi1 = wave_max(hi1); hi2 = wave_max(hi2);
+    bad = wave_max(bad);
+    __syncthreads();
+    if (lane == 0) {
+        red[wave][0] = lo0; red[wave][1] = lo1; red[wave][2] = lo2;
+        red[wave][3] = hi0; red[wave][4] = hi1; red[wave][5] = hi2; red[wave][6] = bad;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    float r[7];
+    for (int k = 0; k < 7; ++k) r[k] = red[0][k];
+    for (int q = 1; q < (int)(blockDim.x >> 6); ++q) {
+        for (int k = 0; k < 3; ++k) r[k] = fminf(r[k], red[q][k]);
+        for (int k = 3; k < 7; ++k) r[k] = fmaxf(r[k], red[q][k]);
     }
     for (int k = 0; k < 6; ++k) S->box[k] = r[k] + 0.0f;
     if (r[6] != 0.0f) atomicOr(&S->status, kStNonfinite);
@@ -229,11 +248,32 @@ __global__ void k_tree_init(int nsys, SysState* st) {
 
 // ---------------------------------------------------------------------------
 // build: lock-free insertion (build_tree.cl:42-161), one body per lane.
-// A lane that finds its slot empty CASes its body in; a lane that finds a body
-// locks the slot (CAS to -2), computes how many cells it takes to separate the
-// two bodies, allocates them in one atomicAdd, writes the chain, and publishes
-// it with a release store.  A lane that finds the slot locked retries on its
-// next iteration.
+// Each round a lane descends through cells and then makes one attempt:
+//   empty slot  -> CAS its body in;
+//   body q      -> CAS the slot to -2 (lock), count the cells k that separate
+//                  the two bodies;
+//   locked slot -> back off, retry next round.
+// After the attempt the wave re-converges, allocates all its new cells with a
+// single atomicAdd (wave prefix sum of k), each lock holder writes its chain
+// with coherent stores, waits for them to complete (s_waitcnt) and publishes
+// the chain by storing the first cell into the locked slot.
+// The descent reads slots with L2-cacheable loads.  A cached value can only be
+// an older value of that slot (-1, a body, or -2; a cell pointer is final),
+// which makes the CAS fail; the retry then re-reads the slot coherently.
+
+__device__ __forceinline__ int ld_w(const int* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ int wave_incl_scan(int x) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
 
 __global__ __launch_bounds__(256) void k_build(const SysDesc* __restrict__ sd, SysState* st,
                                                const BlkWork* __restrict__ work,
@@ -244,6 +284,7 @@ __global__ __launch_bounds__(256) void k_build(const SysDesc* __restrict__ sd, S
     SysState* S = st + w.sys;
     if (S->status) return;
     const int n = d.n;
+    const int lane = threadIdx.x & 63;
     const int b = w.begin + threadIdx.x;
     bool active = b < w.end;
     int* C = child + 8 * (size_t)d.cell_off;
@@ -253,7 +294,7 @@ __global__ __launch_bounds__(256) void k_build(const SysDesc* __restrict__ sd, S
     root_box(d.rules, S->box, rlx, rly, rlz, rhx, rhy, rhz);
     float px = 0.f, py = 0.f, pz = 0.f;
     if (active) {
-        float4 p = node4[d.node_off + b];
+        const float4 p = node4[d.node_off + b];
         px = p.x; py = p.y; pz = p.z;
     }
     float lx = rlx, ly = rly, lz = rlz, hx = rhx, hy = rhy, hz = rhz;
@@ -261,53 +302,72 @@ __global__ __launch_bounds__(256) void k_build(const SysDesc* __restrict__ sd, S
     int dg = level_digit(d.rules, 0, px, py, pz, lx, ly, lz, hx, hy, hz);
     int depth = 1;
     int tries = 0;
-    while (active) {
-        if (++tries > (1 << 22)) {  // cannot happen with a consistent tree; never spin forever
-            atomicOr(&S->status, kStStall);
-            break;
-        }
-        int slot = 8 * node + dg;
-        int v = ld_c(C + slot);
-        while (v >= n) {
-            node = v - n;
-            ++lvl;
-            dg = level_digit(d.rules, lvl, px, py, pz, lx, ly, lz, hx, hy, hz);
-            slot = 8 * node + dg;
-            v = ld_c(C + slot);
-        }
-        if (v == kEmpty) {
-            if (cas_c(C + slot, kEmpty, b)) {
-                depth = max(depth, lvl);
+    bool fresh = false;  // re-read the current slot coherently
+    while (__any(active)) {
+        int k = 0, slot = 0, v = 0;
+        float4 q4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (active) {
+            if (++tries > (1 << 22)) {  // cannot happen on a consistent pool; never spin forever
+                atomicOr(&S->status, kStStall);
                 active = false;
-            }
-        } else if (v >= 0) {
-            if (cas_c(C + slot, v, kLocked)) {
-                const float4 q4 = node4[d.node_off + v];
-                // count the cells needed to separate b and v
-                float ax = lx, ay = ly, az = lz, bx = hx, by = hy, bz = hz;
-                float qx0 = lx, qy0 = ly, qz0 = lz, qx1 = hx, qy1 = hy, qz1 = hz;
-                int k = 0, l = lvl, db, dq;
-                bool deep = false;
-                do {
-                    ++l;
-                    if (l > kMaxLevel) { deep = true; break; }
-                    db = level_digit(d.rules, l, px, py, pz, ax, ay, az, bx, by, bz);
-                    dq = level_digit(d.rules, l, q4.x, q4.y, q4.z, qx0, qy0, qz0, qx1, qy1, qz1);
-                    ++k;
-                } while (db == dq);
-                int err = deep ? kStDepthOverflow : 0;
-                int base = 0;
-                if (!err) {
-                    base = atomicAdd(&S->ncells, k);
-                    if (base + k > d.cap) err = kStTreeOverflow;
+            } else {
+                slot = 8 * node + dg;
+                v = fresh ? ld_c(C + slot) : ld_w(C + slot);
+                while (v >= n) {
+                    node = v - n;
+                    ++lvl;
+                    dg = level_digit(d.rules, lvl, px, py, pz, lx, ly, lz, hx, hy, hz);
+                    slot = 8 * node + dg;
+                    v = ld_w(C + slot);
                 }
-                if (err) {
-                    atomicOr(&S->status, err);
-                    st_c(C + slot, v);  // unlock, leave the old body in place
-                    active = false;
+                fresh = true;
+                if (v == kEmpty) {
+                    if (cas_c(C + slot, kEmpty, b)) {
+                        depth = max(depth, lvl);
+                        active = false;
+                    }
+                } else if (v >= 0) {
+                    if (cas_c(C + slot, v, kLocked)) {
+                        q4 = node4[d.node_off + v];
+                        float ax = lx, ay = ly, az = lz, bx = hx, by = hy, bz = hz;
+                        float qx0 = lx, qy0 = ly, qz0 = lz, qx1 = hx, qy1 = hy, qz1 = hz;
+                        int l = lvl, db, dq;
+                        bool deep = false;
+                        do {
+                            ++l;
+                            if (l > kMaxLevel) { deep = true; break; }
+                            db = level_digit(d.rules, l, px, py, pz, ax, ay, az, bx, by, bz);
+                            dq = level_digit(d.rules, l, q4.x, q4.y, q4.z, qx0, qy0, qz0, qx1, qy1, qz1);
+                            ++k;
+                        } while (db == dq);
+                        if (deep) {
+                            atomicOr(&S->status, kStDepthOverflow);
+                            st_c(C + slot, v);  // unlock, leave the old body in place
+                            k = 0;
+                            active = false;
+                        }
+                    }
                 } else {
-                    ax = lx; ay = ly; az = lz; bx = hx; by = hy; bz = hz;
-                    qx0 = lx; qy0 = ly; qz0 = lz; qx1 = hx; qy1 = hy; qz1 = hz;
+                    __builtin_amdgcn_s_sleep(1);
+                    if (ld_c(&S->status)) active = false;
+                }
+            }
+        }
+        // one allocation for the whole wave
+        const int incl = wave_incl_scan(k);
+        const int total = __shfl(incl, 63, 64);
+        if (total > 0) {
+            int base0 = 0;
+            if (lane == 0) base0 = atomicAdd(&S->ncells, total);
+            base0 = __shfl(base0, 0, 64);
+            if (k > 0) {
+                const int base = base0 + incl - k;
+                if (base + k > d.cap) {
+                    atomicOr(&S->status, kStTreeOverflow);
+                    st_c(C + slot, v);  // unlock, leave the old body in place
+                } else {
+                    float ax = lx, ay = ly, az = lz, bx = hx, by = hy, bz = hz;
+                    float qx0 = lx, qy0 = ly, qz0 = lz, qx1 = hx, qy1 = hy, qz1 = hz;
                     int parent = node;
                     for (int i = 0; i < k; ++i) {
                         const int c = base + i;
@@ -325,18 +385,18 @@ __global__ __launch_bounds__(256) void k_build(const SysDesc* __restrict__ sd, S
                         parent = c;
                     }
                     atomicAdd(pend + node, 1);  // the slot now holds a cell instead of a body
-                    st_rel(C + slot, n + base);
+                    // chain stores are complete (coherent) before the chain becomes reachable
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    __builtin_amdgcn_s_waitcnt(0);
+                    st_c(C + slot, n + base);
                     depth = max(depth, lvl + k);
-                    active = false;
                 }
+                active = false;
             }
-        } else {
-            __builtin_amdgcn_s_sleep(1);
-            if (ld_c(&S->status)) active = false;
         }
     }
     depth = wave_max_i(depth);
-    if ((threadIdx.x & 63) == 0) atomicMax(&S->max_depth, depth);
+    if (lane == 0) atomicMax(&S->max_depth, depth);
 }
 
 // ---------------------------------------------------------------------------
@@ -405,7 +465,11 @@ __global__ __launch_bounds__(256) void k_summarize(const SysDesc* __restrict__ s
         cstart[d.cell_off + c] = -1;
         if (c == 0) break;
         const int p = cparent[d.cell_off + c];
-        const int old = __hip_atomic_fetch_add(cpend + d.cell_off + p, -1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        // this cell's coherent stores complete before the parent's counter drops,
+        // so the lane that takes the counter to zero reads them (sc1 loads)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_waitcnt(0);
+        const int old = __hip_atomic_fetch_add(cpend + d.cell_off + p, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (old != 1) break;
         c = p;
     }
@@ -514,26 +578,30 @@ __global__ __launch_bounds__(256) void k_force_bh(const SysDesc* __restrict__ sd
         const int node = e >> 7;
         const int lvl = min(e & 127, kMaxLevel);
         const float dql = dqs[lvl];
-        const int4* c4 = reinterpret_cast<const int4*>(C + 8 * node);
-        const int4 s0 = c4[0], s1 = c4[1];
-        int vs[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
-        float4 qs[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            vs[k] = __builtin_amdgcn_readfirstlane(vs[k]);
-            qs[k] = nodes[vs[k] < 0 ? 0 : vs[k]];
+        // lanes 0..7 fetch the node's slots and their children's data in one
+        // round trip; the wave then walks the candidates with readlane.
+        int myv = -1;
+        float4 myq = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (lane < 8) {
+            myv = C[8 * node + lane];
+            if (myv >= 0) myq = nodes[myv];
         }
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int v = vs[k];
-            if (v < 0) continue;
-            const float dx = qs[k].x - p.x, dy = qs[k].y - p.y, dz = qs[k].z - p.z;
+        unsigned long long cand = __ballot(myv >= 0);
+        while (cand) {
+            const int j = __builtin_ctzll(cand);
+            cand &= cand - 1;
+            const int v = __builtin_amdgcn_readlane(myv, j);
+            const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.x), j));
+            const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.y), j));
+            const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.z), j));
+            const float qm = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.w), j));
+            const float dx = qx - p.x, dy = qy - p.y, dz = qz - p.z;
             const float d2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, e2)));
             bool take = v < n;
             if (!take) take = __all(d2 >= dql);
             if (take) {
                 const float ri = __builtin_amdgcn_rsqf(d2);
-                const float F = qs[k].w * (ri * ri * ri);
+                const float F = qm * (ri * ri * ri);
                 if (KAHAN) {
                     float y, t;
                     y = dx * F - kx; t = ax + y; kx = (t - ax) - y; ax = t;
