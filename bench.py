@@ -39,6 +39,26 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def calibrate_copy(mib):
+    """A plain device-to-device copy of known size (HIP's blit kernel), so that the
+    HBM byte counters of a profiling run can be scaled: read mib MiB, write mib MiB."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = mib << 20
+    src, dst = C.c_void_p(), C.c_void_p()
+    for ptr in (src, dst):
+        if hip.hipMalloc(C.byref(ptr), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, C.c_size_t(nbytes))
+    for _ in range(3):
+        if hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log(f"bench: copy calibration {mib} MiB x3 (each copy reads and writes {mib} MiB)")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,16 +87,7 @@ def main():
     if info is None:
         raise SystemExit("no HIP device")
     if args.calibrate_copy_mib:
-        import torch  # plumbing only: a copy kernel of known size for counter calibration
-        nel = args.calibrate_copy_mib * (1 << 20) // 4
-        src = torch.ones(nel, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"bench: copy calibration {args.calibrate_copy_mib} MiB x3 (read {args.calibrate_copy_mib} MiB, "
-            f"write {args.calibrate_copy_mib} MiB per launch)")
-        del src, dst
+        calibrate_copy(args.calibrate_copy_mib)
     rules = L.RULES_STANDARD if args.rules == "standard" else L.RULES_REFERENCE
     n, theta, eps, G, dt = args.n, args.theta, 0.01, 1.0, 1e-3
     log(f"bench: device {info}, N={n}, rules={args.rules}, theta={theta}")

@@ -18,13 +18,15 @@ void launch_tree_clear(hipStream_t s, int nblk, const SysDesc* sd, const SysStat
                        int* child, int* cpend);
 void launch_tree_init(hipStream_t s, int nsys, SysState* st);
 void launch_build(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
-                  const float4* node4, int* child, int* cparent, int* cpend);
+                  const float4* node4, int* child, int* cparent, int* cpend, int stride, int exclude);
 void launch_summarize(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
-                      float4* node4, const int* child, const int* cparent, int* cpend, int* ccount, int* cstart);
+                      float4* node4, const int* child, const int* cparent, int* cpend, int* ccount, int* cstart,
+                      float4* cdata);
 void launch_sort(hipStream_t s, int grid, const SysDesc* sd, SysState* st, int nsys, int total_cap,
                  const int* child, const int* ccount, int* cstart, int* sorted);
 void launch_force_bh(hipStream_t s, int nblk, bool kahan, const SysDesc* sd, const SysState* st,
-                     const BlkWork* work, const float4* node4, const int* child, const int* sorted, float4* acc4);
+                     const BlkWork* work, const float4* node4, const int* child, const float4* cdata,
+                     const int* sorted, float4* acc4);
 void launch_brute(hipStream_t s, int nblk, int law, bool kahan, const SysDesc* sd, const SysState* st,
                   const BlkWork* work, const float4* node4, float4* acc4);
 void launch_update(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
@@ -58,12 +60,15 @@ struct Engine {
 
     void* arena = nullptr;
     size_t arena_bytes = 0;
-    float4 *node4 = nullptr, *vel4 = nullptr, *acc4 = nullptr;
+    float4 *node4 = nullptr, *vel4 = nullptr, *acc4 = nullptr, *cdata = nullptr;
     int *sorted = nullptr, *child = nullptr, *cparent = nullptr, *cpend = nullptr, *ccount = nullptr,
         *cstart = nullptr;
     SysDesc* d_desc = nullptr;
     SysState* d_state = nullptr;
     float* d_partial = nullptr;
+    static constexpr int kBuildPhases = 3;
+    BlkWork* t_phase[kBuildPhases] = {};
+    int nb_phase[kBuildPhases] = {};
     BlkWork *t_bounds = nullptr, *t_build = nullptr, *t_cells = nullptr, *t_groups = nullptr, *t_brute = nullptr,
             *t_update = nullptr;
     int nb_bounds = 0, nb_build = 0, nb_cells = 0, nb_groups = 0, nb_brute = 0, nb_update = 0;

@@ -12,6 +12,8 @@ const char* last_error() { return g_err.c_str(); }
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+static constexpr int kPhaseStride[Engine::kBuildPhases] = {1024, 32, 1};
+
 #define NB_HIP(call)                                                                   \
     do {                                                                               \
         hipError_t e_ = (call);                                                        \
@@ -57,6 +59,7 @@ nb_status Engine::init(const std::vector<SysSpec>& systems, void* user_stream) {
     tot_nodes = tot_bodies = tot_cells = 0;
     any_bh = any_bf = false;
     std::vector<BlkWork> tb, tbu, tc, tg, tf, tu;
+    std::vector<BlkWork> tph[kBuildPhases];
     for (int s = 0; s < nsys; ++s) {
         const SysSpec& p = systems[s];
         SysDesc& d = hdesc[s];
@@ -86,6 +89,10 @@ nb_status Engine::init(const std::vector<SysSpec>& systems, void* user_stream) {
             add_ranges(tb, s, d.n, 2048);
             d.bbox_blocks = (int)tb.size() - d.bbox_first;
             add_ranges(tbu, s, d.n, 256);
+            for (int ph = 0; ph < kBuildPhases; ++ph) {
+                const int stride = kPhaseStride[ph];
+                add_ranges(tph[ph], s, (d.n + stride - 1) / stride, 256);
+            }
             add_ranges(tc, s, d.cap, 256);
             add_ranges(tg, s, d.n, 256);
         }
@@ -128,6 +135,10 @@ nb_status Engine::init(const std::vector<SysSpec>& systems, void* user_stream) {
         {(void**)&cpend, sizeof(int) * nc},
         {(void**)&ccount, sizeof(int) * nc},
         {(void**)&cstart, sizeof(int) * nc},
+        {(void**)&cdata, sizeof(float4) * 8 * nc},
+        {(void**)&t_phase[0], sizeof(BlkWork) * std::max<size_t>(tph[0].size(), 1)},
+        {(void**)&t_phase[1], sizeof(BlkWork) * std::max<size_t>(tph[1].size(), 1)},
+        {(void**)&t_phase[2], sizeof(BlkWork) * std::max<size_t>(tph[2].size(), 1)},
         {(void**)&d_desc, sizeof(SysDesc) * std::max(nsys, 1)},
         {(void**)&d_state, sizeof(SysState) * std::max(nsys, 1)},
         {(void**)&d_partial, sizeof(float) * 7 * std::max(nb_bounds, 1)},
@@ -164,6 +175,10 @@ nb_status Engine::init(const std::vector<SysSpec>& systems, void* user_stream) {
     };
     NB_HIP(up(t_bounds, tb));
     NB_HIP(up(t_build, tbu));
+    for (int ph = 0; ph < kBuildPhases; ++ph) {
+        nb_phase[ph] = (int)tph[ph].size();
+        NB_HIP(up(t_phase[ph], tph[ph]));
+    }
     NB_HIP(up(t_cells, tc));
     NB_HIP(up(t_groups, tg));
     NB_HIP(up(t_brute, tf));
@@ -183,17 +198,21 @@ void Engine::tree_reset() {
     launch_tree_clear(stream, nb_cells, d_desc, d_state, t_cells, child, cpend);
     launch_tree_init(stream, nsys, d_state);
 }
+// Insertion phases: every 1024th body, then every 32nd, then the rest.
 void Engine::build() {
-    launch_build(stream, nb_build, d_desc, d_state, t_build, node4, child, cparent, cpend);
+    for (int ph = 0; ph < kBuildPhases; ++ph)
+        launch_build(stream, nb_phase[ph], d_desc, d_state, t_phase[ph], node4, child, cparent, cpend,
+                     kPhaseStride[ph], ph == 0 ? 0 : kPhaseStride[ph - 1]);
 }
 void Engine::summarize() {
-    launch_summarize(stream, nb_cells, d_desc, d_state, t_cells, node4, child, cparent, cpend, ccount, cstart);
+    launch_summarize(stream, nb_cells, d_desc, d_state, t_cells, node4, child, cparent, cpend, ccount, cstart,
+                     cdata);
 }
 void Engine::sort() {
     launch_sort(stream, sort_grid, d_desc, d_state, nsys, (int)tot_cells, child, ccount, cstart, sorted);
 }
 void Engine::force() {
-    launch_force_bh(stream, nb_groups, kahan, d_desc, d_state, t_groups, node4, child, sorted, acc4);
+    launch_force_bh(stream, nb_groups, kahan, d_desc, d_state, t_groups, node4, child, cdata, sorted, acc4);
 }
 void Engine::brute() {
     launch_brute(stream, nb_brute, law, kahan, d_desc, d_state, t_brute, node4, acc4);

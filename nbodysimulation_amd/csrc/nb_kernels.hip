@@ -257,6 +257,10 @@ __global__ void k_tree_init(int nsys, SysState* st) {
 // single atomicAdd (wave prefix sum of k), each lock holder writes its chain
 // with coherent stores, waits for them to complete (s_waitcnt) and publishes
 // the chain by storing the first cell into the locked slot.
+// The build runs in phases over strided subsets of the bodies (every 1024th,
+// then every 32nd, then the rest): the top levels of the tree form with little
+// contention, later phases descend through them.  The topology does not depend
+// on insertion order.
 // The descent reads slots with L2-cacheable loads.  A cached value can only be
 // an older value of that slot (-1, a body, or -2; a cell pointer is final),
 // which makes the CAS fail; the retry then re-reads the slot coherently.
@@ -278,15 +282,16 @@ __device__ __forceinline__ int wave_incl_scan(int x) {
 __global__ __launch_bounds__(256) void k_build(const SysDesc* __restrict__ sd, SysState* st,
                                                const BlkWork* __restrict__ work,
                                                const float4* __restrict__ node4, int* child, int* cparent,
-                                               int* cpend) {
+                                               int* cpend, int stride, int exclude) {
     const BlkWork w = work[blockIdx.x];
     const SysDesc d = sd[w.sys];
     SysState* S = st + w.sys;
     if (S->status) return;
     const int n = d.n;
     const int lane = threadIdx.x & 63;
-    const int b = w.begin + threadIdx.x;
-    bool active = b < w.end;
+    const int bi = w.begin + threadIdx.x;  // index in this phase's strided subset
+    const int b = bi * stride;
+    bool active = bi < w.end && b < n && (exclude == 0 || b % exclude != 0);
     int* C = child + 8 * (size_t)d.cell_off;
     int* par = cparent + d.cell_off;
     int* pend = cpend + d.cell_off;
@@ -401,7 +406,9 @@ __global__ __launch_bounds__(256) void k_build(const SysDesc* __restrict__ sd, S
 
 // ---------------------------------------------------------------------------
 // summarize: bottom-up mass, centre of mass and body count (saturate_tree.cl:6-146)
-// without spin-waits.  A lane starts at a cell whose children are all bodies;
+// without spin-waits.  Each cell also gets its children's (x, y, z, m) in one
+// contiguous 128-byte block (cdata), so the force traversal fetches a cell's
+// slots and all its children's data in a single round trip.  A lane starts at a cell whose children are all bodies;
 // after finishing a cell it decrements the parent's count of unfinished child
 // cells, and the lane that brings it to zero goes on with the parent.  Children
 // are summed in slot order (deterministic; the reference's late children are
@@ -411,7 +418,7 @@ __global__ __launch_bounds__(256) void k_build(const SysDesc* __restrict__ sd, S
 __global__ __launch_bounds__(256) void k_summarize(const SysDesc* __restrict__ sd, const SysState* __restrict__ st,
                                                    const BlkWork* __restrict__ work, float4* node4,
                                                    const int* __restrict__ child, const int* __restrict__ cparent,
-                                                   int* cpend, int* ccount, int* cstart) {
+                                                   int* cpend, int* ccount, int* cstart, float4* cdata) {
     const BlkWork w = work[blockIdx.x];
     const SysDesc d = sd[w.sys];
     const SysState* S = st + w.sys;
@@ -454,6 +461,7 @@ __global__ __launch_bounds__(256) void k_summarize(const SysDesc* __restrict__ s
             cx = fmaf(x, wm, cx);
             cy = fmaf(y, wm, cy);
             cz = fmaf(z, wm, cz);
+            cdata[8 * ((size_t)d.cell_off + c) + k] = make_float4(x, y, z, wm);
         }
         const float inv = 1.0f / m;
         float* o = reinterpret_cast<float*>(nodes + n + c);
@@ -547,7 +555,8 @@ template <bool KAHAN>
 __global__ __launch_bounds__(256) void k_force_bh(const SysDesc* __restrict__ sd, const SysState* __restrict__ st,
                                                   const BlkWork* __restrict__ work,
                                                   const float4* __restrict__ node4, const int* __restrict__ child,
-                                                  const int* __restrict__ sorted, float4* __restrict__ acc4) {
+                                                  const float4* __restrict__ cdata, const int* __restrict__ sorted,
+                                                  float4* __restrict__ acc4) {
     __shared__ int stk[4][kStack];
     __shared__ float dqs[kDqLen];
     const BlkWork w = work[blockIdx.x];
@@ -562,46 +571,45 @@ __global__ __launch_bounds__(256) void k_force_bh(const SysDesc* __restrict__ sd
     const int idx = first + lane;
     const bool valid = idx < w.end;
     const int b = min(max(sorted[d.body_off + (valid ? idx : w.end - 1)], 0), d.n - 1);
-    const float4* __restrict__ nodes = node4 + d.node_off;
-    const float4 p = nodes[b];
+    const float4 p = node4[d.node_off + b];
     const int* __restrict__ C = child + 8 * (size_t)d.cell_off;
+    const float4* __restrict__ CD = cdata + 8 * (size_t)d.cell_off;
     const int n = d.n;
     const float e2 = d.eps * d.eps;
     float ax = 0.f, ay = 0.f, az = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
     int nint = 0;
     int* sk = stk[wave];
+    int sp = 0;
+    // lanes 0..7 hold the current cell's slot k and child k's (x,y,z,m);
+    // the next cell's loads are issued before the current one is processed
     sk[0] = 0;
-    int sp = 1;
+    sp = 1;
     while (sp > 0) {
         --sp;
-        const int e = __builtin_amdgcn_readfirstlane(sk[sp]);
-        const int node = e >> 7;
-        const int lvl = min(e & 127, kMaxLevel);
+        const int cur = __builtin_amdgcn_readfirstlane(sk[sp]);  // (cell << 7) | level
+        const int node = cur >> 7;
+        const int lvl = min(cur & 127, kMaxLevel);
         const float dql = dqs[lvl];
-        // lanes 0..7 fetch the node's slots and their children's data in one
-        // round trip; the wave then walks the candidates with readlane.
-        int myv = -1;
-        float4 myq = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (lane < 8) {
-            myv = C[8 * node + lane];
-            if (myv >= 0) myq = nodes[myv];
-        }
-        unsigned long long cand = __ballot(myv >= 0);
-        while (cand) {
-            const int j = __builtin_ctzll(cand);
-            cand &= cand - 1;
-            const int v = __builtin_amdgcn_readlane(myv, j);
-            const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.x), j));
-            const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.y), j));
-            const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.z), j));
-            const float qm = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(myq.w), j));
-            const float dx = qx - p.x, dy = qy - p.y, dz = qz - p.z;
+        // wave-uniform scalar loads: the 8 slots and the 8 children's (x,y,z,m),
+        // one contiguous 128-byte block per cell
+        const int4* c4 = reinterpret_cast<const int4*>(C + 8 * node);
+        const int4 s0 = c4[0], s1 = c4[1];
+        const int vs[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
+        const float4* qb = CD + 8 * node;
+        float4 qs[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) qs[k] = qb[k];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int v = vs[k];
+            if (v < 0) continue;
+            const float dx = qs[k].x - p.x, dy = qs[k].y - p.y, dz = qs[k].z - p.z;
             const float d2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, e2)));
             bool take = v < n;
             if (!take) take = __all(d2 >= dql);
             if (take) {
                 const float ri = __builtin_amdgcn_rsqf(d2);
-                const float F = qm * (ri * ri * ri);
+                const float F = qs[k].w * (ri * ri * ri);
                 if (KAHAN) {
                     float y, t;
                     y = dx * F - kx; t = ax + y; kx = (t - ax) - y; ax = t;
@@ -728,23 +736,25 @@ void launch_tree_init(hipStream_t s, int nsys, SysState* st) {
     NB_LAUNCH(k_tree_init, (nsys + 255) / 256, 256, s, nsys, st);
 }
 void launch_build(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
-                  const float4* node4, int* child, int* cparent, int* cpend) {
-    NB_LAUNCH(k_build, nblk, 256, s, sd, st, work, node4, child, cparent, cpend);
+                  const float4* node4, int* child, int* cparent, int* cpend, int stride, int exclude) {
+    NB_LAUNCH(k_build, nblk, 256, s, sd, st, work, node4, child, cparent, cpend, stride, exclude);
 }
 void launch_summarize(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
-                      float4* node4, const int* child, const int* cparent, int* cpend, int* ccount, int* cstart) {
-    NB_LAUNCH(k_summarize, nblk, 256, s, sd, st, work, node4, child, cparent, cpend, ccount, cstart);
+                      float4* node4, const int* child, const int* cparent, int* cpend, int* ccount, int* cstart,
+                      float4* cdata) {
+    NB_LAUNCH(k_summarize, nblk, 256, s, sd, st, work, node4, child, cparent, cpend, ccount, cstart, cdata);
 }
 void launch_sort(hipStream_t s, int grid, const SysDesc* sd, SysState* st, int nsys, int total_cap,
                  const int* child, const int* ccount, int* cstart, int* sorted) {
     NB_LAUNCH(k_sort, grid, 256, s, sd, st, nsys, total_cap, child, ccount, cstart, sorted);
 }
 void launch_force_bh(hipStream_t s, int nblk, bool kahan, const SysDesc* sd, const SysState* st,
-                     const BlkWork* work, const float4* node4, const int* child, const int* sorted, float4* acc4) {
+                     const BlkWork* work, const float4* node4, const int* child, const float4* cdata,
+                     const int* sorted, float4* acc4) {
     if (kahan)
-        NB_LAUNCH(k_force_bh<true>, nblk, 256, s, sd, st, work, node4, child, sorted, acc4);
+        NB_LAUNCH(k_force_bh<true>, nblk, 256, s, sd, st, work, node4, child, cdata, sorted, acc4);
     else
-        NB_LAUNCH(k_force_bh<false>, nblk, 256, s, sd, st, work, node4, child, sorted, acc4);
+        NB_LAUNCH(k_force_bh<false>, nblk, 256, s, sd, st, work, node4, child, cdata, sorted, acc4);
 }
 void launch_brute(hipStream_t s, int nblk, int law, bool kahan, const SysDesc* sd, const SysState* st,
                   const BlkWork* work, const float4* node4, float4* acc4) {
