@@ -20,10 +20,10 @@ void launch_tree_init(hipStream_t s, int nsys, SysState* st);
 void launch_build(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
                   const float4* node4, int* child, int* cparent, int* cpend, int stride, int exclude);
 void launch_summarize(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
-                      float4* node4, const int* child, const int* cparent, int* cpend, int* ccount, int* cstart,
+                      float4* node4, const int* child, const int* cparent, int* cpend, int* ccount, int* slotoff,
                       float4* cdata);
-void launch_sort(hipStream_t s, int grid, const SysDesc* sd, SysState* st, int nsys, int total_cap,
-                 const int* child, const int* ccount, int* cstart, int* sorted);
+void launch_sort(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work, const float4* node4,
+                 const int* child, const int* slotoff, int* sorted);
 void launch_force_bh(hipStream_t s, int nblk, bool kahan, const SysDesc* sd, const SysState* st,
                      const BlkWork* work, const float4* node4, const int* child, const float4* cdata,
                      const int* sorted, float4* acc4);
@@ -31,7 +31,6 @@ void launch_brute(hipStream_t s, int nblk, int law, bool kahan, const SysDesc* s
                   const BlkWork* work, const float4* node4, float4* acc4);
 void launch_update(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
                    float4* node4, float4* vel4, const float4* acc4, int mode);
-int sort_blocks_per_cu();
 
 void set_error(const std::string& msg);
 
@@ -62,7 +61,7 @@ struct Engine {
     size_t arena_bytes = 0;
     float4 *node4 = nullptr, *vel4 = nullptr, *acc4 = nullptr, *cdata = nullptr;
     int *sorted = nullptr, *child = nullptr, *cparent = nullptr, *cpend = nullptr, *ccount = nullptr,
-        *cstart = nullptr;
+        *slotoff = nullptr;
     SysDesc* d_desc = nullptr;
     SysState* d_state = nullptr;
     float* d_partial = nullptr;
@@ -72,7 +71,6 @@ struct Engine {
     BlkWork *t_bounds = nullptr, *t_build = nullptr, *t_cells = nullptr, *t_groups = nullptr, *t_brute = nullptr,
             *t_update = nullptr;
     int nb_bounds = 0, nb_build = 0, nb_cells = 0, nb_groups = 0, nb_brute = 0, nb_update = 0;
-    int sort_grid = 0;
 
     ~Engine();
     nb_status init(const std::vector<SysSpec>& systems, void* user_stream);

@@ -91,7 +91,7 @@ nb_status Engine::init(const std::vector<SysSpec>& systems, void* user_stream) {
             add_ranges(tbu, s, d.n, 256);
             for (int ph = 0; ph < kBuildPhases; ++ph) {
                 const int stride = kPhaseStride[ph];
-                add_ranges(tph[ph], s, (d.n + stride - 1) / stride, 256);
+                add_ranges(tph[ph], s, (d.n + stride - 1) / stride, 1024);  // k_build workgroup size
             }
             add_ranges(tc, s, d.cap, 256);
             add_ranges(tg, s, d.n, 256);
@@ -134,7 +134,7 @@ nb_status Engine::init(const std::vector<SysSpec>& systems, void* user_stream) {
         {(void**)&cparent, sizeof(int) * nc},
         {(void**)&cpend, sizeof(int) * nc},
         {(void**)&ccount, sizeof(int) * nc},
-        {(void**)&cstart, sizeof(int) * nc},
+        {(void**)&slotoff, sizeof(int) * 8 * nc},
         {(void**)&cdata, sizeof(float4) * 8 * nc},
         {(void**)&t_phase[0], sizeof(BlkWork) * std::max<size_t>(tph[0].size(), 1)},
         {(void**)&t_phase[1], sizeof(BlkWork) * std::max<size_t>(tph[1].size(), 1)},
@@ -166,7 +166,7 @@ nb_status Engine::init(const std::vector<SysSpec>& systems, void* user_stream) {
     NB_HIP(hipMemsetAsync(cpend, 0, sizeof(int) * nc, stream));
     NB_HIP(hipMemsetAsync(sorted, 0, sizeof(int) * nbd, stream));
     NB_HIP(hipMemsetAsync(ccount, 0, sizeof(int) * nc, stream));
-    NB_HIP(hipMemsetAsync(cstart, 0xff, sizeof(int) * nc, stream));
+    NB_HIP(hipMemsetAsync(slotoff, 0, sizeof(int) * 8 * nc, stream));
     NB_HIP(hipMemsetAsync(d_state, 0, sizeof(SysState) * std::max(nsys, 1), stream));
     NB_HIP(hipMemcpyAsync(d_desc, hdesc.data(), sizeof(SysDesc) * nsys, hipMemcpyHostToDevice, stream));
     auto up = [&](BlkWork* dst, const std::vector<BlkWork>& v) -> hipError_t {
@@ -184,8 +184,6 @@ nb_status Engine::init(const std::vector<SysSpec>& systems, void* user_stream) {
     NB_HIP(up(t_brute, tf));
     NB_HIP(up(t_update, tu));
     launch_tree_init(stream, nsys, d_state);
-    // persistent grid for the top-down sort: half of what fits, so every block is resident
-    sort_grid = std::max(1, sort_blocks_per_cu() * cus / 2);
     NB_HIP(hipGetLastError());
     NB_HIP(hipStreamSynchronize(stream));
     return NB_OK;
@@ -205,11 +203,11 @@ void Engine::build() {
                      kPhaseStride[ph], ph == 0 ? 0 : kPhaseStride[ph - 1]);
 }
 void Engine::summarize() {
-    launch_summarize(stream, nb_cells, d_desc, d_state, t_cells, node4, child, cparent, cpend, ccount, cstart,
+    launch_summarize(stream, nb_cells, d_desc, d_state, t_cells, node4, child, cparent, cpend, ccount, slotoff,
                      cdata);
 }
 void Engine::sort() {
-    launch_sort(stream, sort_grid, d_desc, d_state, nsys, (int)tot_cells, child, ccount, cstart, sorted);
+    launch_sort(stream, nb_build, d_desc, d_state, t_build, node4, child, slotoff, sorted);
 }
 void Engine::force() {
     launch_force_bh(stream, nb_groups, kahan, d_desc, d_state, t_groups, node4, child, cdata, sorted, acc4);

@@ -13,7 +13,9 @@
 //   sorted int[sum n_s]              local body ids in tree order
 //   child  int[8 * sum cap_s]        per cell 8 slots: -1 empty, -2 locked,
 //                                    [0,n) body, n+ci cell ci (local numbering)
-//   cparent, cpend, ccount, cstart   int[sum cap_s]
+//   cparent, cpend, ccount           int[sum cap_s]   parent, unfinished child cells, bodies below
+//   slotoff                          int[8 * sum cap_s] bodies in earlier slots of the same cell
+//   cdata                            float4[8 * sum cap_s] children (x,y,z,m) in slot order
 #pragma once
 #include <stdint.h>
 

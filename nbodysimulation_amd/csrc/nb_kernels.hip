@@ -253,8 +253,9 @@ __global__ void k_tree_init(int nsys, SysState* st) {
 //   body q      -> CAS the slot to -2 (lock), count the cells k that separate
 //                  the two bodies;
 //   locked slot -> back off, retry next round.
-// After the attempt the wave re-converges, allocates all its new cells with a
-// single atomicAdd (wave prefix sum of k), each lock holder writes its chain
+// After the attempt the workgroup re-converges and allocates all its new cells
+// with a single atomicAdd (prefix sums of k over lanes and waves; one counter
+// touched once per workgroup and round instead of once per split), each lock holder writes its chain
 // with coherent stores, waits for them to complete (s_waitcnt) and publishes
 // the chain by storing the first cell into the locked slot.
 // The build runs in phases over strided subsets of the bodies (every 1024th,
@@ -279,7 +280,9 @@ __device__ __forceinline__ int wave_incl_scan(int x) {
     return x;
 }
 
-__global__ __launch_bounds__(256) void k_build(const SysDesc* __restrict__ sd, SysState* st,
+constexpr int kBuildBlock = 1024;  // lanes sharing one allocation atomic per round
+
+__global__ __launch_bounds__(kBuildBlock) void k_build(const SysDesc* __restrict__ sd, SysState* st,
                                                const BlkWork* __restrict__ work,
                                                const float4* __restrict__ node4, int* child, int* cparent,
                                                int* cpend, int stride, int exclude) {
@@ -308,7 +311,12 @@ __global__ __launch_bounds__(256) void k_build(const SysDesc* __restrict__ sd, S
     int depth = 1;
     int tries = 0;
     bool fresh = false;  // re-read the current slot coherently
-    while (__any(active)) {
+    constexpr int kWaves = kBuildBlock / 64;
+    __shared__ int s_wsum[kWaves];
+    __shared__ int s_base;
+    const int wave = threadIdx.x >> 6;
+    // block-synchronous rounds: one attempt per lane, then one allocation for the block
+    while (__syncthreads_or(active)) {
         int k = 0, slot = 0, v = 0;
         float4 q4 = make_float4(0.f, 0.f, 0.f, 0.f);
         if (active) {
@@ -358,50 +366,61 @@ __global__ __launch_bounds__(256) void k_build(const SysDesc* __restrict__ sd, S
                 }
             }
         }
-        // one allocation for the whole wave
+        // one allocation for the whole block: wave prefix sums, then across the waves in LDS
         const int incl = wave_incl_scan(k);
-        const int total = __shfl(incl, 63, 64);
-        if (total > 0) {
-            int base0 = 0;
-            if (lane == 0) base0 = atomicAdd(&S->ncells, total);
-            base0 = __shfl(base0, 0, 64);
-            if (k > 0) {
-                const int base = base0 + incl - k;
-                if (base + k > d.cap) {
-                    atomicOr(&S->status, kStTreeOverflow);
-                    st_c(C + slot, v);  // unlock, leave the old body in place
-                } else {
-                    float ax = lx, ay = ly, az = lz, bx = hx, by = hy, bz = hz;
-                    float qx0 = lx, qy0 = ly, qz0 = lz, qx1 = hx, qy1 = hy, qz1 = hz;
-                    int parent = node;
-                    for (int i = 0; i < k; ++i) {
-                        const int c = base + i;
-                        const int l2 = lvl + 1 + i;
-                        const int db2 = level_digit(d.rules, l2, px, py, pz, ax, ay, az, bx, by, bz);
-                        const int dq2 = level_digit(d.rules, l2, q4.x, q4.y, q4.z, qx0, qy0, qz0, qx1, qy1, qz1);
-                        par[c] = parent;
-                        st_c(pend + c, (i < k - 1) ? 1 : 0);
-                        if (i < k - 1) {
-                            st_c(C + 8 * c + db2, n + c + 1);
-                        } else {
-                            st_c(C + 8 * c + db2, b);
-                            st_c(C + 8 * c + dq2, v);
-                        }
-                        parent = c;
+        if (lane == 63) s_wsum[wave] = incl;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int t = 0;
+            for (int q = 0; q < kWaves; ++q) t += s_wsum[q];
+            s_base = t > 0 ? atomicAdd(&S->ncells, t) : 0;
+        }
+        __syncthreads();
+        if (k > 0) {
+            int woff = 0;
+            for (int q = 0; q < wave; ++q) woff += s_wsum[q];
+            const int base = s_base + woff + incl - k;
+            if (base + k > d.cap) {
+                atomicOr(&S->status, kStTreeOverflow);
+                st_c(C + slot, v);  // unlock, leave the old body in place
+            } else {
+                float ax = lx, ay = ly, az = lz, bx = hx, by = hy, bz = hz;
+                float qx0 = lx, qy0 = ly, qz0 = lz, qx1 = hx, qy1 = hy, qz1 = hz;
+                int parent = node;
+                for (int i = 0; i < k; ++i) {
+                    const int c = base + i;
+                    const int l2 = lvl + 1 + i;
+                    const int db2 = level_digit(d.rules, l2, px, py, pz, ax, ay, az, bx, by, bz);
+                    const int dq2 = level_digit(d.rules, l2, q4.x, q4.y, q4.z, qx0, qy0, qz0, qx1, qy1, qz1);
+                    par[c] = parent;
+                    st_c(pend + c, (i < k - 1) ? 1 : 0);
+                    if (i < k - 1) {
+                        st_c(C + 8 * c + db2, n + c + 1);
+                    } else {
+                        st_c(C + 8 * c + db2, b);
+                        st_c(C + 8 * c + dq2, v);
                     }
-                    atomicAdd(pend + node, 1);  // the slot now holds a cell instead of a body
-                    // chain stores are complete (coherent) before the chain becomes reachable
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-                    __builtin_amdgcn_s_waitcnt(0);
-                    st_c(C + slot, n + base);
-                    depth = max(depth, lvl + k);
+                    parent = c;
                 }
-                active = false;
+                atomicAdd(pend + node, 1);  // the slot now holds a cell instead of a body
+                // chain stores are complete (coherent) before the chain becomes reachable
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_s_waitcnt(0);
+                st_c(C + slot, n + base);
+                depth = max(depth, lvl + k);
             }
+            active = false;
         }
     }
+    // deepest level: block maximum, then an atomic only if it raises the stored value
     depth = wave_max_i(depth);
-    if (lane == 0) atomicMax(&S->max_depth, depth);
+    if (lane == 0) s_wsum[wave] = depth;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int bm = 1;
+        for (int q = 0; q < kWaves; ++q) bm = max(bm, s_wsum[q]);
+        if (bm > ld_c(&S->max_depth)) atomicMax(&S->max_depth, bm);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -418,7 +437,7 @@ __global__ __launch_bounds__(256) void k_build(const SysDesc* __restrict__ sd, S
 __global__ __launch_bounds__(256) void k_summarize(const SysDesc* __restrict__ sd, const SysState* __restrict__ st,
                                                    const BlkWork* __restrict__ work, float4* node4,
                                                    const int* __restrict__ child, const int* __restrict__ cparent,
-                                                   int* cpend, int* ccount, int* cstart, float4* cdata) {
+                                                   int* cpend, int* ccount, int* slotoff, float4* cdata) {
     const BlkWork w = work[blockIdx.x];
     const SysDesc d = sd[w.sys];
     const SysState* S = st + w.sys;
@@ -448,15 +467,18 @@ __global__ __launch_bounds__(256) void k_summarize(const SysDesc* __restrict__ s
             const int v = vs[k];
             if (v < 0) continue;
             float x, y, z, wm;
+            int sub;
             if (v < n) {
                 float4 t = nodes[v];
                 x = t.x; y = t.y; z = t.z; wm = t.w;
-                cnt += 1;
+                sub = 1;
             } else {
                 const float* f = reinterpret_cast<const float*>(nodes + v);
                 x = ld_cf(f); y = ld_cf(f + 1); z = ld_cf(f + 2); wm = ld_cf(f + 3);
-                cnt += ld_c(cnt_base + (v - n));
+                sub = ld_c(cnt_base + (v - n));
             }
+            slotoff[8 * ((size_t)d.cell_off + c) + k] = cnt;  // bodies in earlier slots
+            cnt += sub;
             m = m + wm;
             cx = fmaf(x, wm, cx);
             cy = fmaf(y, wm, cy);
@@ -470,7 +492,6 @@ __global__ __launch_bounds__(256) void k_summarize(const SysDesc* __restrict__ s
         st_cf(o + 2, cz * inv);
         st_cf(o + 3, m);
         st_c(cnt_base + c, cnt);
-        cstart[d.cell_off + c] = -1;
         if (c == 0) break;
         const int p = cparent[d.cell_off + c];
         // this cell's coherent stores complete before the parent's counter drops,
@@ -484,61 +505,42 @@ __global__ __launch_bounds__(256) void k_summarize(const SysDesc* __restrict__ s
 }
 
 // ---------------------------------------------------------------------------
-// sort: tree-order placement (Burtscher & Pingali), top-down.  Every lane walks
-// its cells in increasing cell number; a cell's start offset is published by
-// its parent, which always has a smaller number.  The grid is sized to be fully
-// resident, so the lowest unfinished cell can always proceed.
+// sort: tree-order placement (Burtscher & Pingali's in-order placement, which
+// the reference leaves empty).  A body's rank in depth-first slot order is the
+// sum, along its root-to-leaf path, of the bodies held by earlier slots of each
+// cell on the path; summarize stored those per-slot offsets.  One lane per body
+// re-derives its path (the same octant digits as the build) and adds them up:
+// no waiting, no dependence on the order in which cells were allocated.
 
 __global__ __launch_bounds__(256) void k_sort(const SysDesc* __restrict__ sd, SysState* st,
-                                              int nsys, int total_cap, const int* __restrict__ child,
-                                              const int* __restrict__ ccount, int* cstart, int* sorted) {
-    const int T = gridDim.x * blockDim.x;
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
-    int s = 0;
-    SysDesc d = sd[0];
-    int used = min(st[0].ncells, d.cap);
-    int bad = st[0].status;
-    int waited = 0;
-    while (g < total_cap) {
-        while (g >= d.cell_off + d.cap && s + 1 < nsys) {
-            ++s;
-            d = sd[s];
-            used = min(st[s].ncells, d.cap);
-            bad = st[s].status;
+                                              const BlkWork* __restrict__ work, const float4* __restrict__ node4,
+                                              const int* __restrict__ child, const int* __restrict__ slotoff,
+                                              int* sorted) {
+    const BlkWork w = work[blockIdx.x];
+    const SysDesc d = sd[w.sys];
+    SysState* S = st + w.sys;
+    if (S->status) return;
+    const int b = w.begin + threadIdx.x;
+    if (b >= w.end) return;
+    const int n = d.n;
+    const int* C = child + 8 * (size_t)d.cell_off;
+    const int* O = slotoff + 8 * (size_t)d.cell_off;
+    const float4 p = node4[d.node_off + b];
+    float lx, ly, lz, hx, hy, hz;
+    root_box(d.rules, S->box, lx, ly, lz, hx, hy, hz);
+    int node = 0, rank = 0;
+    for (int lvl = 0; lvl <= kMaxLevel; ++lvl) {
+        const int dg = level_digit(d.rules, lvl, p.x, p.y, p.z, lx, ly, lz, hx, hy, hz);
+        const int v = C[8 * node + dg];
+        rank += O[8 * node + dg];
+        if (v == b) {
+            if (rank < n) sorted[d.body_off + rank] = b;
+            return;
         }
-        const int ci = g - d.cell_off;
-        if (bad || ci >= used) {
-            g += T;
-            continue;
-        }
-        int start = ci == 0 ? 0 : ld_c(cstart + g);
-        if (start < 0) {
-            if (++waited > (1 << 24)) {  // parent never published: inconsistent tree
-                atomicOr(&st[s].status, kStStall);
-                return;
-            }
-            __builtin_amdgcn_s_sleep(2);
-            continue;
-        }
-        waited = 0;
-        const int4* c4 = reinterpret_cast<const int4*>(child + 8 * (size_t)g);
-        int4 a = c4[0], bq = c4[1];
-        int vs[8] = {a.x, a.y, a.z, a.w, bq.x, bq.y, bq.z, bq.w};
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int v = vs[k];
-            if (v < 0) continue;
-            if (v < d.n) {
-                if (start < d.n) sorted[d.body_off + start] = v;
-                start += 1;
-            } else {
-                const int cg = d.cell_off + (v - d.n);
-                st_c(cstart + cg, start);
-                start += ccount[cg];
-            }
-        }
-        g += T;
+        if (v < n) break;  // not this body's slot: inconsistent tree
+        node = v - n;
     }
+    atomicOr(&S->status, kStStall);
 }
 
 // ---------------------------------------------------------------------------
@@ -737,16 +739,16 @@ void launch_tree_init(hipStream_t s, int nsys, SysState* st) {
 }
 void launch_build(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
                   const float4* node4, int* child, int* cparent, int* cpend, int stride, int exclude) {
-    NB_LAUNCH(k_build, nblk, 256, s, sd, st, work, node4, child, cparent, cpend, stride, exclude);
+    NB_LAUNCH(k_build, nblk, kBuildBlock, s, sd, st, work, node4, child, cparent, cpend, stride, exclude);
 }
 void launch_summarize(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
-                      float4* node4, const int* child, const int* cparent, int* cpend, int* ccount, int* cstart,
+                      float4* node4, const int* child, const int* cparent, int* cpend, int* ccount, int* slotoff,
                       float4* cdata) {
-    NB_LAUNCH(k_summarize, nblk, 256, s, sd, st, work, node4, child, cparent, cpend, ccount, cstart, cdata);
+    NB_LAUNCH(k_summarize, nblk, 256, s, sd, st, work, node4, child, cparent, cpend, ccount, slotoff, cdata);
 }
-void launch_sort(hipStream_t s, int grid, const SysDesc* sd, SysState* st, int nsys, int total_cap,
-                 const int* child, const int* ccount, int* cstart, int* sorted) {
-    NB_LAUNCH(k_sort, grid, 256, s, sd, st, nsys, total_cap, child, ccount, cstart, sorted);
+void launch_sort(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work, const float4* node4,
+                 const int* child, const int* slotoff, int* sorted) {
+    NB_LAUNCH(k_sort, nblk, 256, s, sd, st, work, node4, child, slotoff, sorted);
 }
 void launch_force_bh(hipStream_t s, int nblk, bool kahan, const SysDesc* sd, const SysState* st,
                      const BlkWork* work, const float4* node4, const int* child, const float4* cdata,
@@ -768,10 +770,5 @@ void launch_update(hipStream_t s, int nblk, const SysDesc* sd, const SysState* s
     NB_LAUNCH(k_update, nblk, 256, s, sd, st, work, node4, vel4, acc4, mode);
 }
 
-int sort_blocks_per_cu() {
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sort, 256, 0) != hipSuccess) return 1;
-    return nb > 0 ? nb : 1;
-}
 
 }  // namespace nb
