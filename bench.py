@@ -59,8 +59,12 @@ def calibrate_copy(mib):
     log(f"bench: copy calibration {mib} MiB x3 (each copy reads and writes {mib} MiB)")
 
 
-def main():
+def parse_args():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=["bh1m", "ensemble", "precision", "serving", "bf4096"], default="bh1m",
+                    help="bh1m: Barnes-Hut 1M Plummer (headline); ensemble: 512 systems batched vs one per launch; "
+                         "precision: Kahan vs plain brute force against FP64; serving: continuous batching; "
+                         "bf4096: plain brute force N=4096")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -72,12 +76,21 @@ def main():
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (scale for HBM byte counters)")
-    args = ap.parse_args()
+    return ap.parse_args()
+
+
+def main():
+    args = parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each box has one MI355X); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    {"bh1m": bench_bh, "ensemble": bench_ensemble, "precision": bench_precision, "serving": bench_serving,
+     "bf4096": bench_bf4096}[args.workload](args)
+
+
+def bench_bh(args):
 
     from nbodysimulation_amd import _lib as L
     from nbodysimulation_amd import device_info, ic
@@ -189,6 +202,130 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def hip_stream():
+    """A HIP stream for the one-system-per-launch baseline (plumbing via ctypes)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    st = C.c_void_p()
+    if hip.hipStreamCreate(C.byref(st)) != 0:
+        raise RuntimeError("hipStreamCreate failed")
+    return st.value, hip
+
+
+def ensemble_ics(sizes, kinds, seed):
+    from nbodysimulation_amd import ic
+    out = []
+    for i, (n, k) in enumerate(zip(sizes, kinds)):
+        s = seed * 100003 + i
+        if k == 0:
+            p, v, m = ic.plummer(int(n), s)
+        elif k == 1:
+            p, v, m = ic.normal_cloud(int(n), s)
+            m = np.full(int(n), 1.0 / n, np.float32)
+        else:
+            p, v, m = ic.uniform_ball(int(n), s)
+        out.append((p, v, m))
+    return out
+
+
+def bench_ensemble(args):
+    """512 independent systems, N log-uniform in [1024, 65536], each a Plummer, Normal(0,1) or
+    uniform-ball cloud (equal odds), m = 1/N, eps = 0.01, dt = 1e-3, seed 2; brute force below the
+    autotuned crossover N, Barnes-Hut (STANDARD, theta = 0.5) from it on.  One batched, graph-captured
+    step for all systems against the same systems stepped one system per launch."""
+    from nbodysimulation_amd import _lib as L
+    from nbodysimulation_amd import autotune, device_info
+    from nbodysimulation_amd.engine import NBodyEnsemble, NBodySim, make_config, system_spec
+
+    info = device_info()
+    if info is None:
+        raise SystemExit("no HIP device")
+    tune = autotune.load_or_tune(0.5, 0.01, log=log)
+    cross = int(tune["crossover_n"])
+    count = 512
+    sizes = autotune.log_uniform_sizes(count, 1024, 65536, seed=2)
+    kinds = np.random.default_rng(2).integers(0, 3, count)
+    ics = ensemble_ics(sizes, kinds, 2)
+    algos = [autotune.choose_algorithm(int(n), cross) for n in sizes]
+    specs = [system_spec(int(n), a, rules=L.RULES_STANDARD, theta=0.5, eps=0.01, dt=1e-3)
+             for n, a in zip(sizes, algos)]
+    steps, warm = args.steps, args.warmup
+    log(f"bench: ensemble of {count} systems, {int(sizes.sum())} bodies, crossover N={cross}, "
+        f"{sum(a == L.BARNES_HUT for a in algos)} Barnes-Hut / {sum(a != L.BARNES_HUT for a in algos)} brute force")
+
+    with NBodyEnsemble(specs, law=L.LAW_PLUMMER) as ens:
+        for i, (p, v, m) in enumerate(ics):
+            ens.upload(i, p, v, m)
+        ens.step(max(warm, 1))
+        t0 = time.perf_counter()
+        ens.step(steps)
+        t_batched = time.perf_counter() - t0
+        hbm = ens.device_bytes()
+
+    def plain(shared_stream):
+        stream = hip = None
+        if shared_stream:
+            stream, hip = hip_stream()
+        sims = []
+        for sp, (p, v, m) in zip(specs, ics):
+            s = NBodySim(make_config(sp.num_bodies, sp.algorithm, rules=sp.rules, law=L.LAW_PLUMMER, theta=sp.theta,
+                                     eps=sp.epsilon, dt=sp.dt, G=1.0, cell_capacity=2 * sp.num_bodies + 1024),
+                         stream=stream)
+            s.upload(p, v, m)
+            sims.append(s)
+        for s in sims:
+            s.step_async(max(warm, 1))
+        for s in sims:
+            s.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            for s in sims:
+                s.step_async(1)
+        for s in sims:
+            s.sync()
+        t = time.perf_counter() - t0
+        for s in sims:
+            s.close()
+        if hip is not None:
+            hip.hipStreamDestroy(__import__("ctypes").c_void_p(stream))
+        return t
+
+    t_plain_shared = plain(True)
+    t_plain_own = plain(False)
+    bodies = int(sizes.sum())
+    line = {
+        "metric": "ensemble steps/s (512 systems, one batched graph-captured step)",
+        "value": steps / t_batched, "unit": "ensemble steps/s", "n_gpus": 1, "steps": steps, "warmup": warm,
+        "ms_per_step": 1e3 * t_batched / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic: seeded Plummer / Normal(0,1) / unit-ball clouds",
+        "config": {"workload": "512 systems, N log-uniform [1024, 65536], seed 2", "bodies": bodies,
+                   "crossover_n": cross, "barnes_hut_systems": int(sum(a == L.BARNES_HUT for a in algos)),
+                   "hbm_bytes": hbm, "device": info},
+        "bodies_steps_per_s": bodies * steps / t_batched,
+        "plain_one_system_per_launch": {
+            "shared_stream_steps_per_s": steps / t_plain_shared,
+            "per_system_streams_steps_per_s": steps / t_plain_own,
+            "speedup_vs_shared_stream": t_plain_shared / t_batched,
+            "speedup_vs_per_system_streams": t_plain_own / t_batched},
+        "autotune": tune,
+    }
+    log(f"bench: batched {steps / t_batched:.2f} ensemble steps/s; plain shared stream {steps / t_plain_shared:.2f}, "
+        f"own streams {steps / t_plain_own:.2f}")
+    print(json.dumps(line), flush=True)
+
+
+def bench_precision(args):
+    raise SystemExit("precision workload: see below")
+
+
+def bench_serving(args):
+    raise SystemExit("serving workload: not built yet")
+
+
+def bench_bf4096(args):
+    raise SystemExit("bf4096 workload: not built yet")
 
 
 if __name__ == "__main__":

@@ -139,7 +139,7 @@ nb_status nb_stage_sort(nb_sim* sim);
 /* calculate_force_local traversal (kernels/calculate_forces.cl:44-148); writes acc. */
 nb_status nb_stage_force(nb_sim* sim);
 /* Integrator (calculate_forces.cl:150-182): Euler v+=a dt, x+=v dt; under
- * NB_LEAPFROG the closing kick.  Also releases the tree. */
+ * NB_LEAPFROG the closing kick. */
 nb_status nb_stage_update(nb_sim* sim);
 /* update / update_local all-pairs force (kernels/brute_force_update.cl:3-228); writes acc. */
 nb_status nb_stage_brute_force(nb_sim* sim);
@@ -159,6 +159,43 @@ nb_status nb_force_interactions(nb_sim* sim, int64_t* total);
 
 /* Per-stage milliseconds of the last step (needs cfg.time_stages). */
 nb_status nb_stage_times(nb_sim* sim, float* ms);
+/* nsteps whole steps queued on the stream without waiting for them (no status
+ * check; call nb_sim_sync / nb_sim_status afterwards). */
+nb_status nb_step_async(nb_sim* sim, int32_t nsteps);
+
+/* ---------------------------------------------------------------------------
+ * Ensembles: many independent systems stepped together.  The reference steps
+ * one system per frame (CMyApp::Update, src/MyApp.cpp:550-656); here every
+ * stage kernel covers all systems of the batch in one launch (per-system
+ * descriptors and block tables), and a whole ensemble step is one HIP graph.
+ * Each system's results are bit-identical to stepping it alone.
+ * ------------------------------------------------------------------------- */
+typedef struct nb_system_spec {
+    int32_t num_bodies;
+    int32_t algorithm;      /* NB_BARNES_HUT / NB_BRUTE_FORCE_GLOBAL / NB_BRUTE_FORCE_LOCAL */
+    int32_t rules;          /* Barnes-Hut rule set */
+    int32_t cell_capacity;  /* 0: 2N + 1024 */
+    float gravitational_constant;
+    float theta;
+    float epsilon;
+    float dt;
+} nb_system_spec;
+
+typedef struct nb_ensemble nb_ensemble;
+
+/* method, law and accumulation apply to every system of the ensemble. */
+nb_status nb_ens_create(const nb_system_spec* specs, int32_t count, int32_t method, int32_t law,
+                        int32_t accumulation, int32_t use_graph, void* stream, nb_ensemble** out);
+nb_status nb_ens_destroy(nb_ensemble* ens);
+nb_status nb_ens_upload(nb_ensemble* ens, int32_t system, const float* pos_xyz, const float* vel_xyz,
+                        const float* mass);
+nb_status nb_ens_download(nb_ensemble* ens, int32_t system, float* pos_xyz, float* vel_xyz, float* acc_xyz);
+/* nsteps batched steps of every system; synchronises and reports the first failing status. */
+nb_status nb_ens_step(nb_ensemble* ens, int32_t nsteps);
+/* per-system nb_status after the last step (count entries). */
+nb_status nb_ens_status(nb_ensemble* ens, int32_t* status);
+/* HBM bytes the ensemble holds. */
+nb_status nb_ens_device_bytes(nb_ensemble* ens, int64_t* bytes);
 
 #ifdef __cplusplus
 }

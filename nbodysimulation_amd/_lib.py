@@ -70,6 +70,19 @@ class NbTreeInfo(C.Structure):
     ]
 
 
+class NbSystemSpec(C.Structure):
+    _fields_ = [
+        ("num_bodies", C.c_int32),
+        ("algorithm", C.c_int32),
+        ("rules", C.c_int32),
+        ("cell_capacity", C.c_int32),
+        ("gravitational_constant", C.c_float),
+        ("theta", C.c_float),
+        ("epsilon", C.c_float),
+        ("dt", C.c_float),
+    ]
+
+
 class NbIcReferenceParams(C.Structure):
     _fields_ = [
         ("starting_speed_mul", C.c_float),
@@ -133,6 +146,15 @@ SIGNATURES = {
     "nb_export_dq": (C.c_int, [_P, _FP]),
     "nb_force_interactions": (C.c_int, [_P, _I64P]),
     "nb_stage_times": (C.c_int, [_P, _FP]),
+    "nb_step_async": (C.c_int, [_P, C.c_int32]),
+    "nb_ens_create": (C.c_int, [C.POINTER(NbSystemSpec), C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P,
+                                C.POINTER(_P)]),
+    "nb_ens_destroy": (C.c_int, [_P]),
+    "nb_ens_upload": (C.c_int, [_P, C.c_int32, _FP, _FP, _FP]),
+    "nb_ens_download": (C.c_int, [_P, C.c_int32, _FP, _FP, _FP]),
+    "nb_ens_step": (C.c_int, [_P, C.c_int32]),
+    "nb_ens_status": (C.c_int, [_P, _IP]),
+    "nb_ens_device_bytes": (C.c_int, [_P, _I64P]),
     "nb_ic_reference_default": (None, [C.POINTER(NbIcReferenceParams)]),
     "nb_ic_params_default": (None, [C.POINTER(NbIcParams), C.c_int32]),
     "nb_ic_reference": (C.c_int, [C.POINTER(NbIcReferenceParams), C.c_int32, C.c_uint64, _FP, _FP, _FP]),

@@ -479,6 +479,32 @@ nb_status nb_step(nb_sim* sim, int32_t nsteps) {
     return nb_sim_status(sim);
 }
 
+nb_status nb_step_async(nb_sim* sim, int32_t nsteps) {
+    if (!sim || nsteps < 0) return fail(NB_E_INVALID_ARG, "bad arguments");
+    if (!sim->have_state) return fail(NB_E_STATE, "nb_step_async needs uploaded state");
+    if (nsteps == 0 || sim->n == 0) return NB_OK;
+    Engine& e = sim->eng;
+    if (sim->cfg.method == NB_LEAPFROG && !sim->have_acc) {
+        enqueue_force(sim, nullptr);
+        sim->have_acc = true;
+    }
+    if (sim->cfg.use_graph) {
+        if (!sim->graph) {
+            nb_status st = build_graph(sim);
+            if (st) return st;
+        }
+        for (int i = 0; i < nsteps; ++i) NB_HIPC(hipGraphLaunch(sim->graph, e.stream));
+    } else {
+        for (int i = 0; i < nsteps; ++i) enqueue_step(sim, nullptr);
+    }
+    NB_HIPC(hipGetLastError());
+    sim->have_box = false;
+    sim->tree_clean = false;
+    sim->have_tree = sim->have_summary = sim->have_sorted = is_bh(sim);
+    sim->have_acc = true;
+    return NB_OK;
+}
+
 nb_status nb_stage_times(nb_sim* sim, float* ms) {
     if (!sim || !ms) return fail(NB_E_INVALID_ARG, "NULL argument");
     std::memcpy(ms, sim->stage_ms, sizeof(float) * NB_NUM_STAGES);
@@ -636,6 +662,202 @@ nb_status nb_force_interactions(nb_sim* sim, int64_t* total) {
     int64_t s = 0;
     for (int i = 0; i < sim->n; ++i) s += (int64_t)h[i].w;
     *total = s;
+    return NB_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// ensembles
+
+struct nb_ensemble {
+    Engine eng;
+    std::vector<nb_system_spec> specs;
+    int method = NB_EULER;
+    bool use_graph = true;
+    bool have_acc = false;
+    std::vector<bool> uploaded;
+    hipGraphExec_t graph = nullptr;
+    hipGraph_t graph_def = nullptr;
+};
+
+static void ens_enqueue_force(nb_ensemble* e) {
+    Engine& g = e->eng;
+    if (g.any_bh) {
+        g.bounds();
+        g.tree_reset();
+        g.build();
+        g.summarize();
+        g.sort();
+        g.force();
+    }
+    if (g.any_bf) g.brute();
+}
+
+static void ens_enqueue_step(nb_ensemble* e) {
+    const bool lf = e->method == NB_LEAPFROG;
+    if (lf) e->eng.update(2);
+    ens_enqueue_force(e);
+    e->eng.update(lf ? 1 : 0);
+}
+
+extern "C" {
+
+nb_status nb_ens_create(const nb_system_spec* specs, int32_t count, int32_t method, int32_t law,
+                        int32_t accumulation, int32_t use_graph, void* stream, nb_ensemble** out) {
+    if (!out || !specs || count < 1) return fail(NB_E_INVALID_ARG, "bad arguments");
+    *out = nullptr;
+    if (method != NB_EULER && method != NB_LEAPFROG) return fail(NB_E_INVALID_ARG, "bad method");
+    if (law != NB_LAW_PLUMMER && law != NB_LAW_REFERENCE_UNIT) return fail(NB_E_INVALID_ARG, "bad law");
+    if (accumulation != NB_ACCUM_PLAIN && accumulation != NB_ACCUM_KAHAN)
+        return fail(NB_E_INVALID_ARG, "bad accumulation");
+    std::vector<SysSpec> sys(count);
+    for (int i = 0; i < count; ++i) {
+        nb_config c;
+        nb_config_default(&c);
+        c.num_bodies = specs[i].num_bodies;
+        c.algorithm = specs[i].algorithm;
+        c.method = method;
+        c.rules = specs[i].rules;
+        c.law = law;
+        c.accumulation = accumulation;
+        c.gravitational_constant = specs[i].gravitational_constant;
+        c.theta = specs[i].theta;
+        c.epsilon = specs[i].epsilon;
+        c.dt = specs[i].dt;
+        c.cell_capacity = specs[i].cell_capacity;
+        nb_status st = validate(&c);
+        if (st) return fail(st, "system " + std::to_string(i) + ": " + nb::last_error());
+        const int cap = c.algorithm == NB_BARNES_HUT
+                            ? (c.cell_capacity > 0 ? c.cell_capacity : 2 * c.num_bodies + 1024)
+                            : 0;
+        sys[i] = spec_of(&c, cap);
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(NB_E_NO_DEVICE, "no HIP device");
+    nb_ensemble* e = new nb_ensemble();
+    e->specs.assign(specs, specs + count);
+    e->method = method;
+    e->use_graph = use_graph != 0;
+    e->uploaded.assign(count, false);
+    nb_status st = e->eng.init(sys, stream);
+    if (st) {
+        delete e;
+        return st;
+    }
+    *out = e;
+    return NB_OK;
+}
+
+nb_status nb_ens_destroy(nb_ensemble* e) {
+    if (!e) return NB_OK;
+    if (e->eng.stream) (void)hipStreamSynchronize(e->eng.stream);
+    if (e->graph) (void)hipGraphExecDestroy(e->graph);
+    if (e->graph_def) (void)hipGraphDestroy(e->graph_def);
+    delete e;
+    return NB_OK;
+}
+
+nb_status nb_ens_upload(nb_ensemble* e, int32_t sys, const float* pos, const float* vel, const float* mass) {
+    if (!e || sys < 0 || sys >= (int)e->specs.size()) return fail(NB_E_INVALID_ARG, "bad system index");
+    const nb::SysDesc& d = e->eng.hdesc[sys];
+    const int n = d.n;
+    if (n > 0 && (!pos || !mass)) return fail(NB_E_INVALID_ARG, "NULL argument");
+    std::vector<float4> h4(std::max(n, 1)), hv(std::max(n, 1));
+    for (int i = 0; i < n; ++i) {
+        const float m = mass[i];
+        const bool ok = std::isfinite(m) && (d.algo == NB_BARNES_HUT ? m > 0.0f : m >= 0.0f);
+        if (!ok) return fail(NB_E_INVALID_ARG, "mass " + std::to_string(i) + " must be finite and > 0");
+        h4[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], m);
+        hv[i] = vel ? make_float4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], 0.0f) : make_float4(0, 0, 0, 0);
+    }
+    Engine& g = e->eng;
+    if (n > 0) {
+        NB_HIPC(hipMemcpyAsync(g.node4 + d.node_off, h4.data(), sizeof(float4) * n, hipMemcpyHostToDevice, g.stream));
+        NB_HIPC(hipMemcpyAsync(g.vel4 + d.body_off, hv.data(), sizeof(float4) * n, hipMemcpyHostToDevice, g.stream));
+        NB_HIPC(hipMemsetAsync(g.acc4 + d.body_off, 0, sizeof(float4) * n, g.stream));
+    }
+    NB_HIPC(hipStreamSynchronize(g.stream));
+    e->uploaded[sys] = true;
+    e->have_acc = false;
+    return NB_OK;
+}
+
+nb_status nb_ens_download(nb_ensemble* e, int32_t sys, float* pos, float* vel, float* acc) {
+    if (!e || sys < 0 || sys >= (int)e->specs.size()) return fail(NB_E_INVALID_ARG, "bad system index");
+    Engine& g = e->eng;
+    const nb::SysDesc& d = g.hdesc[sys];
+    const int n = d.n;
+    if (n == 0) return NB_OK;
+    std::vector<float4> h(n);
+    auto get = [&](const float4* src, float* dst) -> nb_status {
+        if (!dst) return NB_OK;
+        NB_HIPC(hipMemcpyAsync(h.data(), src, sizeof(float4) * n, hipMemcpyDeviceToHost, g.stream));
+        NB_HIPC(hipStreamSynchronize(g.stream));
+        for (int i = 0; i < n; ++i) {
+            dst[3 * i] = h[i].x;
+            dst[3 * i + 1] = h[i].y;
+            dst[3 * i + 2] = h[i].z;
+        }
+        return NB_OK;
+    };
+    nb_status st = get(g.node4 + d.node_off, pos);
+    if (!st) st = get(g.vel4 + d.body_off, vel);
+    if (!st) st = get(g.acc4 + d.body_off, acc);
+    return st;
+}
+
+nb_status nb_ens_step(nb_ensemble* e, int32_t nsteps) {
+    if (!e || nsteps < 0) return fail(NB_E_INVALID_ARG, "bad arguments");
+    for (bool u : e->uploaded)
+        if (!u) return fail(NB_E_STATE, "every system needs uploaded state");
+    Engine& g = e->eng;
+    if (nsteps == 0) return NB_OK;
+    if (e->method == NB_LEAPFROG && !e->have_acc) {
+        ens_enqueue_force(e);
+        e->have_acc = true;
+    }
+    if (e->use_graph) {
+        if (!e->graph) {
+            NB_HIPC(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
+            ens_enqueue_step(e);
+            hipGraph_t gr = nullptr;
+            hipError_t ce = hipStreamEndCapture(g.stream, &gr);
+            if (ce != hipSuccess) return fail(NB_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+            ce = hipGraphInstantiate(&e->graph, gr, nullptr, nullptr, 0);
+            if (ce != hipSuccess) {
+                (void)hipGraphDestroy(gr);
+                return fail(NB_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(ce));
+            }
+            e->graph_def = gr;
+        }
+        for (int i = 0; i < nsteps; ++i) NB_HIPC(hipGraphLaunch(e->graph, g.stream));
+    } else {
+        for (int i = 0; i < nsteps; ++i) ens_enqueue_step(e);
+    }
+    NB_HIPC(hipGetLastError());
+    NB_HIPC(hipStreamSynchronize(g.stream));
+    e->have_acc = true;
+    std::vector<SysState> s;
+    nb_status st = g.download_state(s);
+    if (st) return st;
+    for (const SysState& x : s)
+        if (x.status) return status_of_bits(x.status);
+    return NB_OK;
+}
+
+nb_status nb_ens_status(nb_ensemble* e, int32_t* status) {
+    if (!e || !status) return fail(NB_E_INVALID_ARG, "NULL argument");
+    std::vector<SysState> s;
+    nb_status st = e->eng.download_state(s);
+    if (st) return st;
+    for (size_t i = 0; i < s.size(); ++i) status[i] = (int32_t)status_of_bits(s[i].status);
+    return NB_OK;
+}
+
+nb_status nb_ens_device_bytes(nb_ensemble* e, int64_t* bytes) {
+    if (!e || !bytes) return fail(NB_E_INVALID_ARG, "NULL argument");
+    *bytes = (int64_t)e->eng.arena_bytes;
     return NB_OK;
 }
 

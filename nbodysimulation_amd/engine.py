@@ -118,6 +118,9 @@ class NBodySim:
     def step(self, nsteps: int = 1):
         self._call("nb_step", int(nsteps))
 
+    def step_async(self, nsteps: int = 1):
+        self._call("nb_step_async", int(nsteps))
+
     # inspection
     def tree_info(self):
         info = L.NbTreeInfo()
@@ -155,6 +158,72 @@ class NBodySim:
         ms = np.zeros(L.NUM_STAGES, np.float32)
         self._call("nb_stage_times", _fp(ms))
         return dict(zip(L.STAGE_NAMES, ms.tolist()))
+
+
+def system_spec(n: int, algorithm: int = L.BARNES_HUT, *, rules: int = L.RULES_STANDARD, G: float = 1.0,
+                theta: float = 0.5, eps: float = 0.01, dt: float = 1e-3, cell_capacity: int = 0) -> L.NbSystemSpec:
+    sp = L.NbSystemSpec()
+    sp.num_bodies, sp.algorithm, sp.rules, sp.cell_capacity = n, algorithm, rules, cell_capacity
+    sp.gravitational_constant, sp.theta, sp.epsilon, sp.dt = G, theta, eps, dt
+    return sp
+
+
+class NBodyEnsemble:
+    """Many independent systems stepped by shared batched launches (one HIP graph per step)."""
+
+    def __init__(self, specs, *, method: int = L.EULER, law: int = L.LAW_PLUMMER, accumulation: int = L.ACCUM_PLAIN,
+                 use_graph: bool = True, stream: int | None = None):
+        self.lib = L.load()
+        self.specs = list(specs)
+        arr = (L.NbSystemSpec * len(self.specs))(*self.specs)
+        h = C.c_void_p()
+        L.check(self.lib.nb_ens_create(arr, len(self.specs), method, law, accumulation, int(use_graph),
+                                       C.c_void_p(stream) if stream else None, C.byref(h)), "nb_ens_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.nb_ens_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, i, pos, vel, mass):
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1)
+        mass = np.ascontiguousarray(mass, dtype=np.float32).reshape(-1)
+        vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float32).reshape(-1)
+        L.check(self.lib.nb_ens_upload(self.h, i, _fp(pos), _fp(vel), _fp(mass)), "nb_ens_upload")
+
+    def download(self, i):
+        n = self.specs[i].num_bodies
+        p = np.empty(3 * n, np.float32)
+        v = np.empty(3 * n, np.float32)
+        a = np.empty(3 * n, np.float32)
+        L.check(self.lib.nb_ens_download(self.h, i, _fp(p), _fp(v), _fp(a)), "nb_ens_download")
+        return p.reshape(-1, 3), v.reshape(-1, 3), a.reshape(-1, 3)
+
+    def step(self, nsteps: int = 1):
+        L.check(self.lib.nb_ens_step(self.h, int(nsteps)), "nb_ens_step")
+
+    def status(self):
+        out = np.zeros(len(self.specs), np.int32)
+        L.check(self.lib.nb_ens_status(self.h, _ip(out)), "nb_ens_status")
+        return out
+
+    def device_bytes(self) -> int:
+        b = C.c_int64()
+        L.check(self.lib.nb_ens_device_bytes(self.h, C.byref(b)), "nb_ens_device_bytes")
+        return b.value
 
 
 def device_info():
