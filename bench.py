@@ -74,6 +74,10 @@ def parse_args():
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--requests", type=int, default=2000, help="serving: number of requests")
+    ap.add_argument("--rate", type=float, default=6.0, help="serving: Poisson arrival rate, requests/s")
+    ap.add_argument("--budget-gib", type=float, default=96.0, help="serving: HBM budget of the server arena")
+    ap.add_argument("--max-systems", type=int, default=512, help="serving: system slots")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (scale for HBM byte counters)")
     return ap.parse_args()
@@ -321,7 +325,106 @@ def bench_precision(args):
 
 
 def bench_serving(args):
-    raise SystemExit("serving workload: not built yet")
+    """Continuous-batching server under open-loop Poisson arrivals.  Each request is a new
+    Plummer system (M = 1, a = 1, seed 3-derived), N log-uniform in [4096, 2,097,152], steps
+    uniform in [10, 200], theta from {0.3, 0.5, 0.7}, eps = 0.01, dt = 1e-3, Euler; brute force
+    below the autotuned crossover N, Barnes-Hut (STANDARD) from it on.  The server admits by
+    HBM budget and steps every resident system once per tick.  Latency runs from the scheduled
+    arrival time to the tick that retires the request; throughput is bodies x steps over the
+    span from the first arrival to the last completion."""
+    import queue
+    import threading
+
+    from nbodysimulation_amd import _lib as L
+    from nbodysimulation_amd import autotune, device_info, ic
+    from nbodysimulation_amd.engine import NBodyServer, system_spec
+
+    info = device_info()
+    if info is None:
+        raise SystemExit("no HIP device")
+    tune = autotune.load_or_tune(0.5, 0.01, log=log)
+    cross = int(tune["crossover_n"])
+    R, rate = args.requests, args.rate
+    rng = np.random.default_rng(3)
+    sizes = autotune.log_uniform_sizes(R, 4096, 2097152, seed=3)
+    steps = rng.integers(10, 201, R)
+    thetas = rng.choice([0.3, 0.5, 0.7], R)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, R))
+    arrivals -= arrivals[0]
+    specs = [system_spec(int(n), autotune.choose_algorithm(int(n), cross), rules=L.RULES_STANDARD,
+                         theta=float(t), eps=0.01, dt=1e-3) for n, t in zip(sizes, thetas)]
+    work = float((sizes * steps).sum())
+    log(f"bench: serving {R} requests at {rate} req/s, {work:.3e} body-steps, budget {args.budget_gib} GiB, "
+        f"{args.max_systems} slots, crossover N={cross}")
+
+    # the client's initial conditions, generated ahead of arrival in a producer thread
+    ready = queue.Queue(maxsize=48)
+
+    def produce():
+        for i in range(R):
+            ready.put(ic.plummer(int(sizes[i]), 3 * 1000003 + i))
+
+    threading.Thread(target=produce, daemon=True).start()
+
+    sv = NBodyServer(int(args.budget_gib * (1 << 30)), args.max_systems)
+    ids = {}
+    finish = np.zeros(R)
+    submit_lag = np.zeros(R)
+    failed = 0
+    nxt = done = 0
+    busy = 0.0
+    t0 = time.perf_counter()
+    last_log = t0
+    while done < R:
+        now = time.perf_counter() - t0
+        while nxt < R and arrivals[nxt] <= now:
+            p, v, m = ready.get()
+            ids[sv.submit(specs[nxt], int(steps[nxt]), p, v, m)] = nxt
+            submit_lag[nxt] = time.perf_counter() - t0 - arrivals[nxt]
+            nxt += 1
+            now = time.perf_counter() - t0
+        st = sv.stats()
+        if st["active"] == 0 and st["queued"] == 0 and st["finished"] == 0:
+            if nxt < R:
+                time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - t0)))
+            continue
+        tk = time.perf_counter()
+        sv.tick()
+        busy += time.perf_counter() - tk
+        while (r := sv.poll()) is not None:
+            i = ids.pop(r[0])
+            finish[i] = time.perf_counter() - t0
+            failed += r[1] != L.NB_OK
+            sv.fetch(r[0])
+            done += 1
+        if time.perf_counter() - last_log > 10:
+            last_log = time.perf_counter()
+            st = sv.stats()
+            log(f"bench: t={last_log - t0:7.1f}s submitted {nxt} done {done} active {st['active']} "
+                f"queued {st['queued']} ticks {st['ticks']}")
+    stats = sv.stats()
+    sv.close()
+    lat = finish - arrivals
+    span = float(finish.max())
+    line = {
+        "metric": "serving bodies*steps/s with p50/p99 request latency (continuous batching)",
+        "value": stats["body_steps"] / span, "unit": "bodies*steps/s", "n_gpus": 1, "steps": int(steps.sum()),
+        "warmup": 0, "ms_per_step": 1e3 * span / max(stats["ticks"], 1), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: seeded Plummer spheres generated by the client while the server runs",
+        "config": {"workload": f"{R} requests, Poisson {rate} req/s, N log-uniform [4096, 2097152], steps U[10,200], "
+                               f"theta in {{0.3,0.5,0.7}}, seed 3", "hbm_budget_bytes": int(args.budget_gib * (1 << 30)),
+                   "arena_bytes": stats["arena_bytes"], "max_systems": args.max_systems, "crossover_n": cross,
+                   "device": info},
+        "latency_s": {"p50": float(np.quantile(lat, 0.5)), "p99": float(np.quantile(lat, 0.99)),
+                      "max": float(lat.max()), "mean": float(lat.mean())},
+        "span_s": span, "ticks": stats["ticks"], "tick_busy_frac": busy / span, "failed": int(failed),
+        "client_submit_lag_s": {"p50": float(np.quantile(submit_lag, 0.5)), "max": float(submit_lag.max())},
+        "offered_body_steps_per_s": work / float(arrivals[-1]) if R > 1 else None,
+    }
+    log(f"bench: serving {line['value']:.3e} bodies*steps/s, latency p50 {line['latency_s']['p50']:.2f}s "
+        f"p99 {line['latency_s']['p99']:.2f}s, span {span:.1f}s, ticks {stats['ticks']}")
+    print(json.dumps(line), flush=True)
 
 
 def bench_bf4096(args):

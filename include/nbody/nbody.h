@@ -197,6 +197,44 @@ nb_status nb_ens_status(nb_ensemble* ens, int32_t* status);
 /* HBM bytes the ensemble holds. */
 nb_status nb_ens_device_bytes(nb_ensemble* ens, int64_t* bytes);
 
+/* ---------------------------------------------------------------------------
+ * Serving: continuous batching of independent simulation requests.  No
+ * counterpart in the reference (one system per process, src/main.cpp); built
+ * on the ensemble kernels.  A server reserves one HBM arena from a byte budget.
+ * A request is a system plus a step count.  Each nb_server_tick admits queued
+ * requests in arrival order while a system slot and arena room are free, runs
+ * one batched step of every resident system, and retires the ones that are
+ * done (or failed), keeping their final state on the host until fetched.
+ * A request's result is bit-identical to stepping its system alone.
+ * Not thread-safe: one caller thread per server.
+ * ------------------------------------------------------------------------- */
+typedef struct nb_server nb_server;
+
+typedef struct nb_server_stats {
+    int32_t active;       /* systems resident in the arena */
+    int32_t queued;       /* submitted, not yet admitted */
+    int32_t finished;     /* retired, not yet fetched */
+    int32_t reserved;
+    int64_t arena_bytes;  /* HBM the server holds */
+    int64_t body_steps;   /* sum over ticks of resident bodies stepped */
+    int64_t ticks;        /* ticks that ran a step */
+} nb_server_stats;
+
+/* method, law and accumulation apply to every request. */
+nb_status nb_server_create(int64_t hbm_budget_bytes, int32_t max_systems, int32_t method, int32_t law,
+                           int32_t accumulation, void* stream, nb_server** out);
+nb_status nb_server_destroy(nb_server* sv);
+/* Queue a request (state is copied).  NB_E_ADMISSION if it can never fit the arena. */
+nb_status nb_server_submit(nb_server* sv, const nb_system_spec* spec, int32_t steps, const float* pos_xyz,
+                           const float* vel_xyz, const float* mass, int64_t* id);
+/* Admit, step, retire; *completed = requests retired by this tick. */
+nb_status nb_server_tick(nb_server* sv, int32_t* completed);
+/* Oldest retired request not yet polled: its id and nb_status.  NB_E_STATE if none. */
+nb_status nb_server_poll(nb_server* sv, int64_t* id, int32_t* status);
+/* Final state of a retired request (any pointer may be NULL); frees it. */
+nb_status nb_server_fetch(nb_server* sv, int64_t id, float* pos_xyz, float* vel_xyz, float* acc_xyz);
+nb_status nb_server_stats_get(nb_server* sv, nb_server_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

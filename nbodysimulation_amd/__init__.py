@@ -10,10 +10,10 @@ from .config import (BARNES_HUT, BRUTE_FORCE_GLOBAL, BRUTE_FORCE_LOCAL, EULER, F
                      RANDOM_VEL, SPHERE_POS, STARTING_IN_VEL, STARTING_OUT_VEL, TANGENT_XZ_VEL, UNIFORM_POS,
                      BarnesHutConfig, EngineOptions, NormalDistribution, Simulation, SimulationConfig,
                      to_nb_config)
-from .engine import NBodyEnsemble, NBodySim, device_info, make_config, system_spec
+from .engine import NBodyEnsemble, NBodyServer, NBodySim, device_info, make_config, system_spec
 
 __all__ = [
-    "_lib", "NbError", "load", "NBodySim", "NBodyEnsemble", "system_spec", "make_config", "device_info", "SimulationConfig", "BarnesHutConfig",
+    "_lib", "NbError", "load", "NBodySim", "NBodyEnsemble", "NBodyServer", "system_spec", "make_config", "device_info", "SimulationConfig", "BarnesHutConfig",
     "Simulation", "NormalDistribution", "EngineOptions", "to_nb_config", "BARNES_HUT", "BRUTE_FORCE_GLOBAL",
     "BRUTE_FORCE_LOCAL", "EULER", "LEAPFROG", "SPHERE_POS", "UNIFORM_POS", "RANDOM_VEL", "STARTING_OUT_VEL",
     "STARTING_IN_VEL", "FUNC_ZERO_VEL", "TANGENT_XZ_VEL",

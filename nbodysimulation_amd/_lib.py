@@ -83,6 +83,18 @@ class NbSystemSpec(C.Structure):
     ]
 
 
+class NbServerStats(C.Structure):
+    _fields_ = [
+        ("active", C.c_int32),
+        ("queued", C.c_int32),
+        ("finished", C.c_int32),
+        ("reserved", C.c_int32),
+        ("arena_bytes", C.c_int64),
+        ("body_steps", C.c_int64),
+        ("ticks", C.c_int64),
+    ]
+
+
 class NbIcReferenceParams(C.Structure):
     _fields_ = [
         ("starting_speed_mul", C.c_float),
@@ -155,6 +167,13 @@ SIGNATURES = {
     "nb_ens_step": (C.c_int, [_P, C.c_int32]),
     "nb_ens_status": (C.c_int, [_P, _IP]),
     "nb_ens_device_bytes": (C.c_int, [_P, _I64P]),
+    "nb_server_create": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, C.POINTER(_P)]),
+    "nb_server_destroy": (C.c_int, [_P]),
+    "nb_server_submit": (C.c_int, [_P, C.POINTER(NbSystemSpec), C.c_int32, _FP, _FP, _FP, _I64P]),
+    "nb_server_tick": (C.c_int, [_P, _IP]),
+    "nb_server_poll": (C.c_int, [_P, _I64P, _IP]),
+    "nb_server_fetch": (C.c_int, [_P, C.c_int64, _FP, _FP, _FP]),
+    "nb_server_stats_get": (C.c_int, [_P, C.POINTER(NbServerStats)]),
     "nb_ic_reference_default": (None, [C.POINTER(NbIcReferenceParams)]),
     "nb_ic_params_default": (None, [C.POINTER(NbIcParams), C.c_int32]),
     "nb_ic_reference": (C.c_int, [C.POINTER(NbIcReferenceParams), C.c_int32, C.c_uint64, _FP, _FP, _FP]),

@@ -713,25 +713,8 @@ nb_status nb_ens_create(const nb_system_spec* specs, int32_t count, int32_t meth
         return fail(NB_E_INVALID_ARG, "bad accumulation");
     std::vector<SysSpec> sys(count);
     for (int i = 0; i < count; ++i) {
-        nb_config c;
-        nb_config_default(&c);
-        c.num_bodies = specs[i].num_bodies;
-        c.algorithm = specs[i].algorithm;
-        c.method = method;
-        c.rules = specs[i].rules;
-        c.law = law;
-        c.accumulation = accumulation;
-        c.gravitational_constant = specs[i].gravitational_constant;
-        c.theta = specs[i].theta;
-        c.epsilon = specs[i].epsilon;
-        c.dt = specs[i].dt;
-        c.cell_capacity = specs[i].cell_capacity;
-        nb_status st = validate(&c);
+        nb_status st = nb::spec_from_system(specs[i], method, law, accumulation, &sys[i]);
         if (st) return fail(st, "system " + std::to_string(i) + ": " + nb::last_error());
-        const int cap = c.algorithm == NB_BARNES_HUT
-                            ? (c.cell_capacity > 0 ? c.cell_capacity : 2 * c.num_bodies + 1024)
-                            : 0;
-        sys[i] = spec_of(&c, cap);
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(NB_E_NO_DEVICE, "no HIP device");
@@ -862,3 +845,30 @@ nb_status nb_ens_device_bytes(nb_ensemble* e, int64_t* bytes) {
 }
 
 }  // extern "C"
+
+namespace nb {
+
+nb_status status_from_bits(int bits) { return status_of_bits(bits); }
+
+nb_status spec_from_system(const nb_system_spec& s, int method, int law, int accum, SysSpec* out) {
+    nb_config c;
+    nb_config_default(&c);
+    c.num_bodies = s.num_bodies;
+    c.algorithm = s.algorithm;
+    c.method = method;
+    c.rules = s.rules;
+    c.law = law;
+    c.accumulation = accum;
+    c.gravitational_constant = s.gravitational_constant;
+    c.theta = s.theta;
+    c.epsilon = s.epsilon;
+    c.dt = s.dt;
+    c.cell_capacity = s.cell_capacity;
+    nb_status st = validate(&c);
+    if (st) return st;
+    const int cap = c.algorithm == NB_BARNES_HUT ? (c.cell_capacity > 0 ? c.cell_capacity : 2 * c.num_bodies + 1024) : 0;
+    *out = spec_of(&c, cap);
+    return NB_OK;
+}
+
+}  // namespace nb

@@ -1,5 +1,10 @@
 // nb_engine.h -- host runtime of the N-body engine: device arena, block tables,
 // stage launches for a batch of independent systems.
+//
+// The arena is reserved once (capacities in nodes, bodies, cells and system
+// slots); `activate` then describes which systems occupy which ranges and
+// rebuilds the block tables, so systems can join and leave (serving) without
+// reallocating device memory.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -16,7 +21,7 @@ void launch_bounds(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, con
                    const float4* node4, float* partial);
 void launch_tree_clear(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
                        int* child, int* cpend);
-void launch_tree_init(hipStream_t s, int nsys, SysState* st);
+void launch_tree_init(hipStream_t s, int nsys, const int* slots, SysState* st);  // slots NULL: 0..nsys-1
 void launch_build(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
                   const float4* node4, int* child, int* cparent, int* cpend, int stride, int exclude);
 void launch_summarize(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
@@ -33,26 +38,45 @@ void launch_update(hipStream_t s, int nblk, const SysDesc* sd, const SysState* s
                    float4* node4, float4* vel4, const float4* acc4, int mode);
 
 void set_error(const std::string& msg);
+const char* last_error();
 
-// One system of a batch, as requested by the caller.
+// One system, as requested by the caller.
 struct SysSpec {
     int n = 0;
-    int cap = 0;  // cells (BH only)
+    int cap = 0;  // cells (Barnes-Hut only)
     int rules = 0, algo = 0, law = 1, accum = 0, method = 2;
     float G = 1.f, theta = 0.5f, eps = 0.01f, dt = 1e-3f;
 };
 
-// Device-side batch of systems with its block tables.
+// A system placed in the arena.
+struct Slot {
+    int slot = 0;  // index into the SysDesc / SysState arrays
+    SysSpec spec;
+    long long node_off = 0, body_off = 0, cell_off = 0;
+};
+
+// Arena capacities.
+struct Capacity {
+    long long nodes = 0, bodies = 0, cells = 0;
+    int systems = 0;
+};
+
+// Bytes of device memory per node, body and cell in the arena layout.
+constexpr size_t kBytesPerNode = 16;                     // node4
+constexpr size_t kBytesPerBody = 16 + 16 + 4;            // vel4, acc4, sorted
+constexpr size_t kBytesPerCell = 32 + 4 * 3 + 32 + 128;  // child, parent/pend/count, slotoff, cdata
+
 struct Engine {
+    static constexpr int kBuildPhases = 3;
+
     int device = 0;
     int cus = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
 
-    std::vector<SysSpec> spec;
-    std::vector<SysDesc> hdesc;
-    int nsys = 0;
-    long long tot_nodes = 0, tot_bodies = 0, tot_cells = 0;
+    Capacity capc;
+    std::vector<SysDesc> hdesc;  // indexed by slot
+    std::vector<Slot> active;
     bool any_bh = false, any_bf = false;
     int law = 1;
     bool kahan = false;
@@ -65,14 +89,24 @@ struct Engine {
     SysDesc* d_desc = nullptr;
     SysState* d_state = nullptr;
     float* d_partial = nullptr;
-    static constexpr int kBuildPhases = 3;
-    BlkWork* t_phase[kBuildPhases] = {};
-    int nb_phase[kBuildPhases] = {};
-    BlkWork *t_bounds = nullptr, *t_build = nullptr, *t_cells = nullptr, *t_groups = nullptr, *t_brute = nullptr,
-            *t_update = nullptr;
-    int nb_bounds = 0, nb_build = 0, nb_cells = 0, nb_groups = 0, nb_brute = 0, nb_update = 0;
+    int* d_slots = nullptr;  // slots of the active systems (tree_init touches only these)
+    int nactive = 0;
+
+    // block tables (device) with their capacities and current lengths
+    enum Table { T_BOUNDS, T_BODY, T_CELLS, T_GROUPS, T_BRUTE, T_UPDATE, T_PHASE0, T_PHASE1, T_PHASE2, T_COUNT };
+    BlkWork* tab[T_COUNT] = {};
+    int tab_cap[T_COUNT] = {};
+    int tab_len[T_COUNT] = {};
 
     ~Engine();
+    // Reserve the arena; `law`/`kahan` apply to every system.
+    nb_status reserve(const Capacity& c, int law_, bool kahan_, void* user_stream);
+    // Describe the systems now resident (their ranges must lie inside the arena)
+    // and rebuild the block tables.  Synchronous on the engine's stream.
+    nb_status activate(const std::vector<Slot>& slots);
+    // Prepare a newly placed system: clear its cell slots and reset its state.
+    nb_status reset_slot(const Slot& s);
+    // Static batch: reserve exactly what `systems` need and place them in order.
     nb_status init(const std::vector<SysSpec>& systems, void* user_stream);
     void release();
 
@@ -90,5 +124,10 @@ struct Engine {
 };
 
 size_t align_up(size_t x, size_t a);
+
+// nb_api.hip: validation of one system of a batch, and the nb_status of a
+// system's device status bits (also sets the last-error text).
+nb_status spec_from_system(const nb_system_spec& s, int method, int law, int accum, SysSpec* out);
+nb_status status_from_bits(int bits);
 
 }  // namespace nb

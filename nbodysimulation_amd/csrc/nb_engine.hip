@@ -1,4 +1,4 @@
-// nb_engine.hip -- host runtime: one HBM arena per batch, block tables, stage launches.
+// nb_engine.hip -- host runtime: one HBM arena, per-slot descriptors, block tables, stage launches.
 #include "nb_engine.h"
 
 #include <algorithm>
@@ -12,7 +12,10 @@ const char* last_error() { return g_err.c_str(); }
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+// Insertion phases of the build: every 1024th body, then every 32nd, then the rest.
 static constexpr int kPhaseStride[Engine::kBuildPhases] = {1024, 32, 1};
+static constexpr int kBuildBlock = 1024;  // k_build workgroup size
+static constexpr int kBoundsPerBlock = 2048;
 
 #define NB_HIP(call)                                                                   \
     do {                                                                               \
@@ -33,16 +36,24 @@ void Engine::release() {
     own_stream = false;
 }
 
-// Block table over `count` items per system, `per` items per workgroup.
+// Block table over `count` items of one system, `per` items per workgroup.
 static void add_ranges(std::vector<BlkWork>& t, int sys, int count, int per) {
     for (int b = 0; b < count; b += per) t.push_back(BlkWork{sys, b, std::min(count, b + per), 0});
 }
 
-nb_status Engine::init(const std::vector<SysSpec>& systems, void* user_stream) {
+static int blocks_bound(long long items, int per, int systems) {
+    return (int)std::min<long long>((items + per - 1) / per + 2LL * systems, 0x7fffffff);
+}
+
+nb_status Engine::reserve(const Capacity& c, int law_, bool kahan_, void* user_stream) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
         set_error("no HIP device");
         return NB_E_NO_DEVICE;
+    }
+    if (c.nodes > 0x7fffffffLL || c.bodies > 0x7fffffffLL || c.cells * 8 > 0x7fffffffLL * 4LL || c.systems < 1) {
+        set_error("arena too large for 32-bit numbering, or no system slots");
+        return NB_E_INVALID_ARG;
     }
     NB_HIP(hipGetDevice(&device));
     NB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
@@ -53,102 +64,40 @@ nb_status Engine::init(const std::vector<SysSpec>& systems, void* user_stream) {
         NB_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         own_stream = true;
     }
-    spec = systems;
-    nsys = (int)systems.size();
-    hdesc.assign(nsys, SysDesc{});
-    tot_nodes = tot_bodies = tot_cells = 0;
-    any_bh = any_bf = false;
-    std::vector<BlkWork> tb, tbu, tc, tg, tf, tu;
-    std::vector<BlkWork> tph[kBuildPhases];
-    for (int s = 0; s < nsys; ++s) {
-        const SysSpec& p = systems[s];
-        SysDesc& d = hdesc[s];
-        const bool bh = p.algo == NB_BARNES_HUT;
-        d.n = p.n;
-        d.cap = bh ? p.cap : 0;
-        d.node_off = (int)tot_nodes;
-        d.body_off = (int)tot_bodies;
-        d.cell_off = (int)tot_cells;
-        d.rules = p.rules;
-        d.algo = p.algo;
-        d.law = p.law;
-        d.accum = p.accum;
-        d.method = p.method;
-        d.G = p.G;
-        d.theta = p.theta;
-        d.eps = p.eps;
-        d.dt = p.dt;
-        d.ueps = p.algo == NB_BRUTE_FORCE_LOCAL ? 0.1f : 0.01f;
-        tot_nodes += (long long)d.n + d.cap;
-        tot_bodies += d.n;
-        tot_cells += d.cap;
-        if (bh && d.n > 0) {
-            any_bh = true;
-            d.bbox_first = (int)tb.size();
-            // ~2048 bodies per bounds workgroup (8 per lane)
-            add_ranges(tb, s, d.n, 2048);
-            d.bbox_blocks = (int)tb.size() - d.bbox_first;
-            add_ranges(tbu, s, d.n, 256);
-            for (int ph = 0; ph < kBuildPhases; ++ph) {
-                const int stride = kPhaseStride[ph];
-                add_ranges(tph[ph], s, (d.n + stride - 1) / stride, 1024);  // k_build workgroup size
-            }
-            add_ranges(tc, s, d.cap, 256);
-            add_ranges(tg, s, d.n, 256);
-        }
-        if (!bh && d.n > 0) {
-            any_bf = true;
-            add_ranges(tf, s, d.n, 256);
-        }
-        add_ranges(tu, s, d.n, 256);
-        law = p.law;
-        kahan = p.accum == NB_ACCUM_KAHAN;
-    }
-    if (tot_nodes > 0x7fffffffLL || tot_cells * 8 > 0x7fffffffLL * 4LL) {
-        set_error("batch too large for 32-bit node numbering");
-        return NB_E_INVALID_ARG;
-    }
-    for (const SysDesc& d : hdesc) {
-        if (d.cap >= (1 << 24)) {
-            set_error("cell capacity must be below 2^24 per system");
-            return NB_E_INVALID_ARG;
-        }
-    }
-    nb_bounds = (int)tb.size();
-    nb_build = (int)tbu.size();
-    nb_cells = (int)tc.size();
-    nb_groups = (int)tg.size();
-    nb_brute = (int)tf.size();
-    nb_update = (int)tu.size();
+    capc = c;
+    law = law_;
+    kahan = kahan_;
+    hdesc.assign(c.systems, SysDesc{});
+    const int S = c.systems;
+    tab_cap[T_BOUNDS] = blocks_bound(c.bodies, kBoundsPerBlock, S);
+    tab_cap[T_BODY] = blocks_bound(c.bodies, 256, S);
+    tab_cap[T_CELLS] = blocks_bound(c.cells, 256, S);
+    tab_cap[T_GROUPS] = blocks_bound(c.bodies, 256, S);
+    tab_cap[T_BRUTE] = blocks_bound(c.bodies, 256, S);
+    tab_cap[T_UPDATE] = blocks_bound(c.bodies, 256, S);
+    for (int ph = 0; ph < kBuildPhases; ++ph)
+        tab_cap[T_PHASE0 + ph] = blocks_bound((c.bodies + kPhaseStride[ph] - 1) / kPhaseStride[ph], kBuildBlock, S);
 
-    // one arena, 256-byte aligned regions
     struct Region { void** dst; size_t bytes; };
-    const size_t nn = (size_t)std::max<long long>(tot_nodes, 1), nbd = (size_t)std::max<long long>(tot_bodies, 1),
-                 nc = (size_t)std::max<long long>(tot_cells, 1);
+    const size_t nn = (size_t)std::max<long long>(c.nodes, 1), nb = (size_t)std::max<long long>(c.bodies, 1),
+                 nc = (size_t)std::max<long long>(c.cells, 1);
     std::vector<Region> regs = {
         {(void**)&node4, sizeof(float4) * nn},
-        {(void**)&vel4, sizeof(float4) * nbd},
-        {(void**)&acc4, sizeof(float4) * nbd},
-        {(void**)&sorted, sizeof(int) * nbd},
+        {(void**)&vel4, sizeof(float4) * nb},
+        {(void**)&acc4, sizeof(float4) * nb},
+        {(void**)&sorted, sizeof(int) * nb},
         {(void**)&child, sizeof(int) * 8 * nc},
         {(void**)&cparent, sizeof(int) * nc},
         {(void**)&cpend, sizeof(int) * nc},
         {(void**)&ccount, sizeof(int) * nc},
         {(void**)&slotoff, sizeof(int) * 8 * nc},
         {(void**)&cdata, sizeof(float4) * 8 * nc},
-        {(void**)&t_phase[0], sizeof(BlkWork) * std::max<size_t>(tph[0].size(), 1)},
-        {(void**)&t_phase[1], sizeof(BlkWork) * std::max<size_t>(tph[1].size(), 1)},
-        {(void**)&t_phase[2], sizeof(BlkWork) * std::max<size_t>(tph[2].size(), 1)},
-        {(void**)&d_desc, sizeof(SysDesc) * std::max(nsys, 1)},
-        {(void**)&d_state, sizeof(SysState) * std::max(nsys, 1)},
-        {(void**)&d_partial, sizeof(float) * 7 * std::max(nb_bounds, 1)},
-        {(void**)&t_bounds, sizeof(BlkWork) * std::max(nb_bounds, 1)},
-        {(void**)&t_build, sizeof(BlkWork) * std::max(nb_build, 1)},
-        {(void**)&t_cells, sizeof(BlkWork) * std::max(nb_cells, 1)},
-        {(void**)&t_groups, sizeof(BlkWork) * std::max(nb_groups, 1)},
-        {(void**)&t_brute, sizeof(BlkWork) * std::max(nb_brute, 1)},
-        {(void**)&t_update, sizeof(BlkWork) * std::max(nb_update, 1)},
+        {(void**)&d_desc, sizeof(SysDesc) * S},
+        {(void**)&d_state, sizeof(SysState) * S},
+        {(void**)&d_partial, sizeof(float) * 7 * (size_t)tab_cap[T_BOUNDS]},
+        {(void**)&d_slots, sizeof(int) * S},
     };
+    for (int t = 0; t < T_COUNT; ++t) regs.push_back({(void**)&tab[t], sizeof(BlkWork) * (size_t)tab_cap[t]});
     size_t total = 0;
     for (auto& r : regs) total = align_up(total, 256) + r.bytes;
     arena_bytes = align_up(total, 256);
@@ -160,68 +109,169 @@ nb_status Engine::init(const std::vector<SysSpec>& systems, void* user_stream) {
         off += r.bytes;
     }
     NB_HIP(hipMemsetAsync(node4, 0, sizeof(float4) * nn, stream));
-    NB_HIP(hipMemsetAsync(vel4, 0, sizeof(float4) * nbd, stream));
-    NB_HIP(hipMemsetAsync(acc4, 0, sizeof(float4) * nbd, stream));
+    NB_HIP(hipMemsetAsync(vel4, 0, sizeof(float4) * nb, stream));
+    NB_HIP(hipMemsetAsync(acc4, 0, sizeof(float4) * nb, stream));
+    NB_HIP(hipMemsetAsync(sorted, 0, sizeof(int) * nb, stream));
     NB_HIP(hipMemsetAsync(child, 0xff, sizeof(int) * 8 * nc, stream));
     NB_HIP(hipMemsetAsync(cpend, 0, sizeof(int) * nc, stream));
-    NB_HIP(hipMemsetAsync(sorted, 0, sizeof(int) * nbd, stream));
     NB_HIP(hipMemsetAsync(ccount, 0, sizeof(int) * nc, stream));
     NB_HIP(hipMemsetAsync(slotoff, 0, sizeof(int) * 8 * nc, stream));
-    NB_HIP(hipMemsetAsync(d_state, 0, sizeof(SysState) * std::max(nsys, 1), stream));
-    NB_HIP(hipMemcpyAsync(d_desc, hdesc.data(), sizeof(SysDesc) * nsys, hipMemcpyHostToDevice, stream));
-    auto up = [&](BlkWork* dst, const std::vector<BlkWork>& v) -> hipError_t {
-        if (v.empty()) return hipSuccess;
-        return hipMemcpyAsync(dst, v.data(), sizeof(BlkWork) * v.size(), hipMemcpyHostToDevice, stream);
-    };
-    NB_HIP(up(t_bounds, tb));
-    NB_HIP(up(t_build, tbu));
-    for (int ph = 0; ph < kBuildPhases; ++ph) {
-        nb_phase[ph] = (int)tph[ph].size();
-        NB_HIP(up(t_phase[ph], tph[ph]));
-    }
-    NB_HIP(up(t_cells, tc));
-    NB_HIP(up(t_groups, tg));
-    NB_HIP(up(t_brute, tf));
-    NB_HIP(up(t_update, tu));
-    launch_tree_init(stream, nsys, d_state);
+    NB_HIP(hipMemsetAsync(d_state, 0, sizeof(SysState) * S, stream));
+    launch_tree_init(stream, S, nullptr, d_state);
     NB_HIP(hipGetLastError());
     NB_HIP(hipStreamSynchronize(stream));
     return NB_OK;
 }
 
+nb_status Engine::activate(const std::vector<Slot>& slots) {
+    std::vector<BlkWork> t[T_COUNT];
+    std::vector<int> ids;
+    std::vector<char> seen(capc.systems, 0);
+    any_bh = any_bf = false;
+    for (const Slot& sl : slots) {
+        if (sl.slot < 0 || sl.slot >= capc.systems || seen[sl.slot]) {
+            set_error("slot index out of range or used twice");
+            return NB_E_INVALID_ARG;
+        }
+        seen[sl.slot] = 1;
+        const SysSpec& p = sl.spec;
+        const bool bh = p.algo == NB_BARNES_HUT;
+        if (sl.node_off + p.n + (bh ? p.cap : 0) > capc.nodes || sl.body_off + p.n > capc.bodies ||
+            sl.cell_off + (bh ? p.cap : 0) > capc.cells) {
+            set_error("system range outside the arena");
+            return NB_E_INVALID_ARG;
+        }
+        if (bh && p.cap >= (1 << 24)) {
+            set_error("cell capacity must be below 2^24 per system");
+            return NB_E_INVALID_ARG;
+        }
+        SysDesc& d = hdesc[sl.slot];
+        d = SysDesc{};
+        d.n = p.n;
+        d.cap = bh ? p.cap : 0;
+        d.node_off = (int)sl.node_off;
+        d.body_off = (int)sl.body_off;
+        d.cell_off = (int)sl.cell_off;
+        d.rules = p.rules;
+        d.algo = p.algo;
+        d.law = p.law;
+        d.accum = p.accum;
+        d.method = p.method;
+        d.G = p.G;
+        d.theta = p.theta;
+        d.eps = p.eps;
+        d.dt = p.dt;
+        d.ueps = p.algo == NB_BRUTE_FORCE_LOCAL ? 0.1f : 0.01f;
+        const int s = sl.slot;
+        ids.push_back(s);
+        if (bh && d.n > 0) {
+            any_bh = true;
+            d.bbox_first = (int)t[T_BOUNDS].size();
+            add_ranges(t[T_BOUNDS], s, d.n, kBoundsPerBlock);
+            d.bbox_blocks = (int)t[T_BOUNDS].size() - d.bbox_first;
+            add_ranges(t[T_BODY], s, d.n, 256);
+            add_ranges(t[T_CELLS], s, d.cap, 256);
+            add_ranges(t[T_GROUPS], s, d.n, 256);
+            for (int ph = 0; ph < kBuildPhases; ++ph)
+                add_ranges(t[T_PHASE0 + ph], s, (d.n + kPhaseStride[ph] - 1) / kPhaseStride[ph], kBuildBlock);
+        }
+        if (!bh && d.n > 0) {
+            any_bf = true;
+            add_ranges(t[T_BRUTE], s, d.n, 256);
+        }
+        add_ranges(t[T_UPDATE], s, d.n, 256);
+    }
+    for (int k = 0; k < T_COUNT; ++k) {
+        if ((int)t[k].size() > tab_cap[k]) {
+            set_error("block table capacity exceeded");
+            return NB_E_INVALID_ARG;
+        }
+        tab_len[k] = (int)t[k].size();
+        if (!t[k].empty())
+            NB_HIP(hipMemcpyAsync(tab[k], t[k].data(), sizeof(BlkWork) * t[k].size(), hipMemcpyHostToDevice, stream));
+    }
+    NB_HIP(hipMemcpyAsync(d_desc, hdesc.data(), sizeof(SysDesc) * capc.systems, hipMemcpyHostToDevice, stream));
+    if (!ids.empty()) NB_HIP(hipMemcpyAsync(d_slots, ids.data(), sizeof(int) * ids.size(), hipMemcpyHostToDevice, stream));
+    NB_HIP(hipStreamSynchronize(stream));
+    nactive = (int)ids.size();
+    active = slots;
+    return NB_OK;
+}
+
+nb_status Engine::reset_slot(const Slot& s) {
+    const bool bh = s.spec.algo == NB_BARNES_HUT;
+    if (bh && s.spec.cap > 0) {
+        NB_HIP(hipMemsetAsync(child + 8 * s.cell_off, 0xff, sizeof(int) * 8 * (size_t)s.spec.cap, stream));
+        NB_HIP(hipMemsetAsync(cpend + s.cell_off, 0, sizeof(int) * (size_t)s.spec.cap, stream));
+    }
+    SysState z;
+    std::memset(&z, 0, sizeof(z));
+    z.ncells = 1;
+    z.max_depth = 1;
+    NB_HIP(hipMemcpyAsync(d_state + s.slot, &z, sizeof(SysState), hipMemcpyHostToDevice, stream));
+    NB_HIP(hipStreamSynchronize(stream));
+    return NB_OK;
+}
+
+nb_status Engine::init(const std::vector<SysSpec>& systems, void* user_stream) {
+    Capacity c;
+    c.systems = std::max<int>((int)systems.size(), 1);
+    std::vector<Slot> slots;
+    for (size_t i = 0; i < systems.size(); ++i) {
+        const SysSpec& p = systems[i];
+        const int cap = p.algo == NB_BARNES_HUT ? p.cap : 0;
+        Slot s;
+        s.slot = (int)i;
+        s.spec = p;
+        s.spec.cap = cap;
+        s.node_off = c.nodes;
+        s.body_off = c.bodies;
+        s.cell_off = c.cells;
+        c.nodes += (long long)p.n + cap;
+        c.bodies += p.n;
+        c.cells += cap;
+        slots.push_back(s);
+    }
+    const int law_ = systems.empty() ? 1 : systems.back().law;
+    const bool kahan_ = !systems.empty() && systems.back().accum == NB_ACCUM_KAHAN;
+    nb_status st = reserve(c, law_, kahan_, user_stream);
+    if (st) return st;
+    return activate(slots);
+}
+
 void Engine::bounds() {
-    launch_bounds(stream, nb_bounds, d_desc, d_state, t_bounds, node4, d_partial);
+    launch_bounds(stream, tab_len[T_BOUNDS], d_desc, d_state, tab[T_BOUNDS], node4, d_partial);
 }
 void Engine::tree_reset() {
-    launch_tree_clear(stream, nb_cells, d_desc, d_state, t_cells, child, cpend);
-    launch_tree_init(stream, nsys, d_state);
+    launch_tree_clear(stream, tab_len[T_CELLS], d_desc, d_state, tab[T_CELLS], child, cpend);
+    launch_tree_init(stream, nactive, d_slots, d_state);
 }
-// Insertion phases: every 1024th body, then every 32nd, then the rest.
 void Engine::build() {
     for (int ph = 0; ph < kBuildPhases; ++ph)
-        launch_build(stream, nb_phase[ph], d_desc, d_state, t_phase[ph], node4, child, cparent, cpend,
-                     kPhaseStride[ph], ph == 0 ? 0 : kPhaseStride[ph - 1]);
+        launch_build(stream, tab_len[T_PHASE0 + ph], d_desc, d_state, tab[T_PHASE0 + ph], node4, child, cparent,
+                     cpend, kPhaseStride[ph], ph == 0 ? 0 : kPhaseStride[ph - 1]);
 }
 void Engine::summarize() {
-    launch_summarize(stream, nb_cells, d_desc, d_state, t_cells, node4, child, cparent, cpend, ccount, slotoff,
-                     cdata);
+    launch_summarize(stream, tab_len[T_CELLS], d_desc, d_state, tab[T_CELLS], node4, child, cparent, cpend, ccount,
+                     slotoff, cdata);
 }
 void Engine::sort() {
-    launch_sort(stream, nb_build, d_desc, d_state, t_build, node4, child, slotoff, sorted);
+    launch_sort(stream, tab_len[T_BODY], d_desc, d_state, tab[T_BODY], node4, child, slotoff, sorted);
 }
 void Engine::force() {
-    launch_force_bh(stream, nb_groups, kahan, d_desc, d_state, t_groups, node4, child, cdata, sorted, acc4);
+    launch_force_bh(stream, tab_len[T_GROUPS], kahan, d_desc, d_state, tab[T_GROUPS], node4, child, cdata, sorted,
+                    acc4);
 }
 void Engine::brute() {
-    launch_brute(stream, nb_brute, law, kahan, d_desc, d_state, t_brute, node4, acc4);
+    launch_brute(stream, tab_len[T_BRUTE], law, kahan, d_desc, d_state, tab[T_BRUTE], node4, acc4);
 }
 void Engine::update(int mode) {
-    launch_update(stream, nb_update, d_desc, d_state, t_update, node4, vel4, acc4, mode);
+    launch_update(stream, tab_len[T_UPDATE], d_desc, d_state, tab[T_UPDATE], node4, vel4, acc4, mode);
 }
 
 nb_status Engine::download_state(std::vector<SysState>& out) {
-    out.resize(nsys);
-    NB_HIP(hipMemcpyAsync(out.data(), d_state, sizeof(SysState) * nsys, hipMemcpyDeviceToHost, stream));
+    out.resize(capc.systems);
+    NB_HIP(hipMemcpyAsync(out.data(), d_state, sizeof(SysState) * capc.systems, hipMemcpyDeviceToHost, stream));
     NB_HIP(hipStreamSynchronize(stream));
     return NB_OK;
 }

@@ -239,9 +239,10 @@ __global__ __launch_bounds__(256) void k_tree_clear(const SysDesc* __restrict__ 
     cpend[d.cell_off + ci] = 0;
 }
 
-__global__ void k_tree_init(int nsys, SysState* st) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nsys) return;
+__global__ void k_tree_init(int nsys, const int* __restrict__ slots, SysState* st) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nsys) return;
+    const int s = slots ? slots[k] : k;
     st[s].ncells = 1;
     st[s].max_depth = 1;  // build_tree.cl:28
 }
@@ -734,8 +735,9 @@ void launch_tree_clear(hipStream_t s, int nblk, const SysDesc* sd, const SysStat
                        int* child, int* cpend) {
     NB_LAUNCH(k_tree_clear, nblk, 256, s, sd, st, work, child, cpend);
 }
-void launch_tree_init(hipStream_t s, int nsys, SysState* st) {
-    NB_LAUNCH(k_tree_init, (nsys + 255) / 256, 256, s, nsys, st);
+void launch_tree_init(hipStream_t s, int nsys, const int* slots, SysState* st) {
+    if (nsys <= 0) return;
+    NB_LAUNCH(k_tree_init, (nsys + 255) / 256, 256, s, nsys, slots, st);
 }
 void launch_build(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
                   const float4* node4, int* child, int* cparent, int* cpend, int stride, int exclude) {

@@ -226,6 +226,76 @@ class NBodyEnsemble:
         return b.value
 
 
+class NBodyServer:
+    """Continuous-batching server: requests (system + step count) join and leave
+    one shared batched step per tick inside a fixed HBM arena."""
+
+    def __init__(self, hbm_budget_bytes: int, max_systems: int, *, method: int = L.EULER,
+                 law: int = L.LAW_PLUMMER, accumulation: int = L.ACCUM_PLAIN, stream: int | None = None):
+        self.lib = L.load()
+        h = C.c_void_p()
+        L.check(self.lib.nb_server_create(int(hbm_budget_bytes), int(max_systems), method, law, accumulation,
+                                          C.c_void_p(stream) if stream else None, C.byref(h)), "nb_server_create")
+        self.h = h
+        self.sizes = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.nb_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, spec, steps: int, pos, vel, mass) -> int:
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1)
+        mass = np.ascontiguousarray(mass, dtype=np.float32).reshape(-1)
+        vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float32).reshape(-1)
+        rid = C.c_int64()
+        L.check(self.lib.nb_server_submit(self.h, C.byref(spec), int(steps), _fp(pos), _fp(vel), _fp(mass),
+                                          C.byref(rid)), "nb_server_submit")
+        self.sizes[rid.value] = spec.num_bodies
+        return rid.value
+
+    def tick(self) -> int:
+        done = C.c_int32()
+        L.check(self.lib.nb_server_tick(self.h, C.byref(done)), "nb_server_tick")
+        return done.value
+
+    def poll(self):
+        """(id, status) of the oldest retired request, or None."""
+        rid, st = C.c_int64(), C.c_int32()
+        if self.lib.nb_server_poll(self.h, C.byref(rid), C.byref(st)) != L.NB_OK:
+            return None
+        return rid.value, st.value
+
+    def fetch(self, rid: int):
+        """Final (pos, vel, acc) of a retired request; the server forgets it afterwards."""
+        if rid not in self.sizes:  # not submitted here, or fetched already: the library reports it
+            L.check(self.lib.nb_server_fetch(self.h, int(rid), None, None, None), "nb_server_fetch")
+        n = self.sizes.get(rid, 0)
+        p = np.empty(3 * n, np.float32)
+        v = np.empty(3 * n, np.float32)
+        a = np.empty(3 * n, np.float32)
+        L.check(self.lib.nb_server_fetch(self.h, int(rid), _fp(p), _fp(v), _fp(a)), "nb_server_fetch")
+        self.sizes.pop(rid, None)
+        return p.reshape(-1, 3), v.reshape(-1, 3), a.reshape(-1, 3)
+
+    def stats(self) -> dict:
+        s = L.NbServerStats()
+        L.check(self.lib.nb_server_stats_get(self.h, C.byref(s)), "nb_server_stats_get")
+        return {k: getattr(s, k) for k, _ in L.NbServerStats._fields_ if k != "reserved"}
+
+
 def device_info():
     lib = L.load()
     cu = C.c_int32()
