@@ -66,7 +66,8 @@ def parse_args():
                          "precision: Kahan vs plain brute force against FP64; serving: continuous batching; "
                          "bf4096: plain brute force N=4096")
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: bh1m 20, ensemble 20, bf4096 100, precision 200)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--rules", choices=["standard", "reference"], default="standard")
@@ -83,8 +84,13 @@ def parse_args():
     return ap.parse_args()
 
 
+DEFAULT_STEPS = {"bh1m": 20, "ensemble": 20, "bf4096": 100, "precision": 200, "serving": 0}
+
+
 def main():
     args = parse_args()
+    if args.steps is None:
+        args.steps = DEFAULT_STEPS[args.workload]
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each box has one MI355X); "
@@ -321,7 +327,87 @@ def bench_ensemble(args):
 
 
 def bench_precision(args):
-    raise SystemExit("precision workload: see below")
+    """Brute force at N = 131,072: two Plummer spheres of 65,536 equal-mass bodies (a = 0.5,
+    M = 1 each, centres +-(2,0,0), bulk velocities -+(0.3,0,0)), eps = 0.005, G = 1, dt = 1e-3,
+    Euler, seed 3.  Plain and Kahan-compensated FP32 accumulation: force error against an FP64
+    direct sum on 1024 seeded targets, then 200 steps each with FP64 energy / momentum /
+    angular momentum computed on the GPU before and after."""
+    from nbodysimulation_amd import _lib as L
+    from nbodysimulation_amd import device_info, ic
+    from nbodysimulation_amd.engine import NBodySim, make_config
+    from oracle import oracle as orc
+
+    info = device_info()
+    if info is None:
+        raise SystemExit("no HIP device")
+    n, eps, G, dt, steps = 131072, 0.005, 1.0, 1e-3, args.steps
+    pos, vel, mass = ic.collision(n, 3)
+    tg = np.random.default_rng(12345).choice(n, 1024, replace=False).astype(np.int32)
+    a64 = orc.direct_f64(pos, mass, L.LAW_PLUMMER, eps, G, targets=tg)
+    out = {}
+    for name, accum in (("plain", L.ACCUM_PLAIN), ("kahan", L.ACCUM_KAHAN)):
+        cfg = dict(law=L.LAW_PLUMMER, accumulation=accum, eps=eps, G=G, dt=dt)
+        with NBodySim(make_config(n, L.BRUTE_FORCE_GLOBAL, **cfg)) as s:
+            s.upload(pos, vel, mass)
+            s.brute_force()
+            _, _, a = s.download()
+            rel = np.linalg.norm(a[tg] - a64, axis=1) / np.linalg.norm(a64, axis=1)
+            e0 = s.energy()
+            s.step(1)  # graph capture outside the timed run
+            s.upload(pos, vel, mass)
+            t0 = time.perf_counter()
+            s.step(steps)
+            wall = time.perf_counter() - t0
+            e1 = s.energy()
+        # kernel time from HIP events on the simulation's stream (a separate short run)
+        with NBodySim(make_config(n, L.BRUTE_FORCE_GLOBAL, time_stages=True, **cfg)) as s:
+            s.upload(pos, vel, mass)
+            s.step(args.warmup or 1)
+            s.step(10)
+            brute_ms = s.stage_times()["brute"] / 10
+        pscale = float(np.abs(mass[:, None] * vel).sum())
+        lscale = float((mass * np.linalg.norm(pos, axis=1) * np.linalg.norm(vel, axis=1)).sum())
+        out[name] = {
+            "force_rel_err_vs_fp64": {"median": float(np.median(rel)), "p99": float(np.quantile(rel, 0.99)),
+                                      "max": float(rel.max()), "targets": 1024},
+            "steps_per_s": steps / wall, "brute_kernel_ms": brute_ms,
+            "energy_start": e0["total"], "energy_end": e1["total"],
+            "energy_rel_drift": abs(e1["total"] - e0["total"]) / abs(e0["total"]),
+            "momentum_drift_rel": float(np.linalg.norm(e1["momentum"] - e0["momentum"]) / pscale),
+            "angular_momentum_drift_rel": float(np.linalg.norm(e1["angular"] - e0["angular"]) / lscale),
+        }
+        log(f"bench: {name}: median err {out[name]['force_rel_err_vs_fp64']['median']:.3e}, "
+            f"{steps / wall:.1f} steps/s, brute {brute_ms:.3f} ms, dE/E {out[name]['energy_rel_drift']:.3e}")
+
+    # CPU baseline: the oracle's brute force on 2048 sampled targets, scaled to N
+    cpu = None
+    if not args.no_cpu_baseline:
+        ts = np.random.default_rng(5).choice(n, 2048, replace=False).astype(np.int32)
+        t0 = time.perf_counter()
+        orc.force_bf(pos, mass, L.LAW_PLUMMER, eps, G, targets=ts)
+        t_cpu = (time.perf_counter() - t0) * n / ts.size
+        cpu = {"value": 1.0 / t_cpu, "unit": "timesteps/s", "cores": orc.threads(), "kind": "port",
+               "sample": f"oracle brute-force force pass (plain FP32) on 2048 of {n} targets, scaled to {n}; "
+                         f"{t_cpu:.2f} s per step"}
+    flops = FLOPS_PER_INTERACTION * float(n) * n
+    achieved = flops / (out["plain"]["brute_kernel_ms"] * 1e-3) / 1e12
+    line = {
+        "metric": "brute-force timesteps/s at N=131072 (plain FP32; Kahan and FP64 diagnostics beside)",
+        "value": out["plain"]["steps_per_s"], "unit": "timesteps/s", "n_gpus": 1, "steps": steps,
+        "warmup": 1, "ms_per_step": 1e3 / out["plain"]["steps_per_s"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: two seeded Plummer spheres (library generator)",
+        "config": {"workload": f"brute force, two colliding Plummer spheres N={n} eps={eps} G={G} dt={dt} Euler "
+                               f"seed 3", "device": info},
+        "ginteractions_per_s": float(n) * n * out["plain"]["steps_per_s"] / 1e9,
+        "plain": out["plain"], "kahan": out["kahan"],
+        "roofline": {"bound": "mfma", "note": "FP32 vector ALU (no MFMA); peak = MI355X FP32 157.3 TFLOP/s",
+                     "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel": "k_brute",
+                     "kernel_ms": out["plain"]["brute_kernel_ms"], "flops_per_launch": flops},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
 
 
 def bench_serving(args):
@@ -428,7 +514,67 @@ def bench_serving(args):
 
 
 def bench_bf4096(args):
-    raise SystemExit("bf4096 workload: not built yet")
+    """Plain brute force, N = 4096: x ~ Normal(0,1) per axis, m ~ U[0.5,1.5]/N, v = 0, G = 1,
+    dt = 1e-3, Euler, seed 0, the reference update kernel's law (unit vectors, e = 0.01); the
+    Plummer-softened law (eps = 0.01) is timed beside it.  Steps replay a captured HIP graph."""
+    from nbodysimulation_amd import _lib as L
+    from nbodysimulation_amd import device_info, ic
+    from nbodysimulation_amd.engine import NBodySim, make_config
+    from oracle import oracle as orc
+
+    info = device_info()
+    if info is None:
+        raise SystemExit("no HIP device")
+    n, dt, steps = 4096, 1e-3, args.steps
+    pos, vel, mass = ic.normal_cloud(n, 0)
+    res = {}
+    for name, law in (("reference_unit", L.LAW_REFERENCE_UNIT), ("plummer", L.LAW_PLUMMER)):
+        cfg = dict(law=law, eps=0.01, G=1.0, dt=dt)
+        with NBodySim(make_config(n, L.BRUTE_FORCE_GLOBAL, **cfg)) as s:
+            s.upload(pos, vel, mass)
+            s.step(max(args.warmup, 1))
+            s.upload(pos, vel, mass)
+            t0 = time.perf_counter()
+            s.step(steps)
+            wall = time.perf_counter() - t0
+        with NBodySim(make_config(n, L.BRUTE_FORCE_GLOBAL, time_stages=True, **cfg)) as s:
+            s.upload(pos, vel, mass)
+            s.step(max(args.warmup, 1))
+            s.step(steps)
+            st = s.stage_times()
+        res[name] = {"steps_per_s": steps / wall, "brute_kernel_ms": st["brute"] / steps,
+                     "update_kernel_ms": st["update"] / steps}
+        log(f"bench: bf4096 {name}: {steps / wall:.1f} steps/s, brute {st['brute'] / steps * 1e3:.2f} us")
+    cpu = None
+    if not args.no_cpu_baseline:
+        p, v = pos.copy(), vel.copy()
+        k = 20
+        t0 = time.perf_counter()
+        for _ in range(k):
+            a, _ = orc.force_bf(p, mass, orc.LAW_REFERENCE_UNIT, 0.01, 1.0)
+            v, p = orc.euler(a, v, p, dt)
+        t_cpu = (time.perf_counter() - t0) / k
+        cpu = {"value": 1.0 / t_cpu, "unit": "timesteps/s", "cores": orc.threads(), "kind": "port",
+               "sample": f"{k} full steps of the same workload on the oracle (all-pairs force + Euler)"}
+    r = res["reference_unit"]
+    flops = FLOPS_PER_INTERACTION * float(n) * n
+    achieved = flops / (r["brute_kernel_ms"] * 1e-3) / 1e12
+    line = {
+        "metric": "brute-force timesteps/s at N=4096 (reference update-kernel law)",
+        "value": r["steps_per_s"], "unit": "timesteps/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 / r["steps_per_s"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic: seeded Normal(0,1) cloud, m ~ U[0.5,1.5]/N (library generator)",
+        "config": {"workload": "plain brute force N=4096, Normal(0,1), G=1, dt=1e-3, 100 steps, seed 0",
+                   "device": info},
+        "laws": res,
+        "roofline": {"bound": "mfma", "note": "FP32 vector ALU (no MFMA); peak = MI355X FP32 157.3 TFLOP/s; "
+                                              "at N=4096 one launch is a few microseconds",
+                     "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel": "k_brute",
+                     "kernel_ms": r["brute_kernel_ms"], "flops_per_launch": flops},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

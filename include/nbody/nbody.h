@@ -163,6 +163,15 @@ nb_status nb_stage_times(nb_sim* sim, float* ms);
  * check; call nb_sim_sync / nb_sim_status afterwards). */
 nb_status nb_step_async(nb_sim* sim, int32_t nsteps);
 
+/* FP64 conservation diagnostics of the current state, computed on the GPU (no
+ * counterpart in the reference).  out[NB_ENERGY_LEN] = kinetic, potential,
+ * px, py, pz, Lx, Ly, Lz, total mass.  The pair potential is the one the
+ * configured force law derives from: PLUMMER (and Barnes-Hut)
+ * -G m_i m_j / sqrt(r^2 + eps^2); REFERENCE_UNIT +G m_i m_j r / (1 + ueps^2).
+ * O(N^2) FP64 work; synchronous. */
+#define NB_ENERGY_LEN 9
+nb_status nb_sim_energy(nb_sim* sim, double* out);
+
 /* ---------------------------------------------------------------------------
  * Ensembles: many independent systems stepped together.  The reference steps
  * one system per frame (CMyApp::Update, src/MyApp.cpp:550-656); here every

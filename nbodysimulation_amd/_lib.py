@@ -125,6 +125,7 @@ class NbIcParams(C.Structure):
 _P = C.c_void_p
 _FP = C.POINTER(C.c_float)
 _IP = C.POINTER(C.c_int32)
+_DP = C.POINTER(C.c_double)
 _I64P = C.POINTER(C.c_int64)
 _U8P = C.POINTER(C.c_uint8)
 
@@ -159,6 +160,7 @@ SIGNATURES = {
     "nb_force_interactions": (C.c_int, [_P, _I64P]),
     "nb_stage_times": (C.c_int, [_P, _FP]),
     "nb_step_async": (C.c_int, [_P, C.c_int32]),
+    "nb_sim_energy": (C.c_int, [_P, _DP]),
     "nb_ens_create": (C.c_int, [C.POINTER(NbSystemSpec), C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P,
                                 C.POINTER(_P)]),
     "nb_ens_destroy": (C.c_int, [_P]),

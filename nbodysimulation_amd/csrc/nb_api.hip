@@ -34,6 +34,7 @@ struct nb_sim {
     hipGraph_t graph_def = nullptr;
     std::vector<hipEvent_t> ev;  // 8 per timed step
     float stage_ms[NB_NUM_STAGES] = {};
+    double* energy_buf = nullptr;  // per-workgroup partial sums + result of nb_sim_energy
 };
 
 #define NB_HIPC(call)                                                                  \
@@ -189,6 +190,7 @@ nb_status nb_sim_destroy(nb_sim* sim) {
     drop_graph(sim);
     for (auto& e : sim->ev)
         if (e) (void)hipEventDestroy(e);
+    if (sim->energy_buf) (void)hipFree(sim->energy_buf);
     delete sim;
     return NB_OK;
 }
@@ -502,6 +504,29 @@ nb_status nb_step_async(nb_sim* sim, int32_t nsteps) {
     sim->tree_clean = false;
     sim->have_tree = sim->have_summary = sim->have_sorted = is_bh(sim);
     sim->have_acc = true;
+    return NB_OK;
+}
+
+nb_status nb_sim_energy(nb_sim* sim, double* out) {
+    if (!sim || !out) return fail(NB_E_INVALID_ARG, "NULL argument");
+    if (!sim->have_state) return fail(NB_E_STATE, "no state uploaded");
+    Engine& g = sim->eng;
+    const int n = sim->n;
+    if (n == 0) {
+        std::fill(out, out + NB_ENERGY_LEN, 0.0);
+        return NB_OK;
+    }
+    const int nblk = (n + 255) / 256;
+    if (!sim->energy_buf)
+        NB_HIPC(hipMalloc(&sim->energy_buf, sizeof(double) * nb::kEnergyLen * (size_t)(nblk + 1)));
+    const bool plummer = sim->cfg.algorithm == NB_BARNES_HUT || sim->cfg.law == NB_LAW_PLUMMER;
+    const double eps = plummer ? sim->cfg.epsilon : (sim->cfg.algorithm == NB_BRUTE_FORCE_LOCAL ? 0.1 : 0.01);
+    double* res = sim->energy_buf + (size_t)nb::kEnergyLen * nblk;
+    nb::launch_energy(g.stream, n, g.node4, g.vel4, plummer ? 0 : 1, eps, sim->cfg.gravitational_constant,
+                      sim->energy_buf, res);
+    NB_HIPC(hipGetLastError());
+    NB_HIPC(hipMemcpyAsync(out, res, sizeof(double) * nb::kEnergyLen, hipMemcpyDeviceToHost, g.stream));
+    NB_HIPC(hipStreamSynchronize(g.stream));
     return NB_OK;
 }
 

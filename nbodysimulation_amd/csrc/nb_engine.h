@@ -32,8 +32,10 @@ void launch_sort(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const
 void launch_force_bh(hipStream_t s, int nblk, bool kahan, const SysDesc* sd, const SysState* st,
                      const BlkWork* work, const float4* node4, const int* child, const float4* cdata,
                      const int* sorted, float4* acc4);
-void launch_brute(hipStream_t s, int nblk, int law, bool kahan, const SysDesc* sd, const SysState* st,
+void launch_brute(hipStream_t s, int nblk, int law, bool kahan, int tb, const SysDesc* sd, const SysState* st,
                   const BlkWork* work, const float4* node4, float4* acc4);
+void launch_energy(hipStream_t s, int n, const float4* node4, const float4* vel4, int law, double eps, double G,
+                   double* partial, double* out);
 void launch_update(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
                    float4* node4, float4* vel4, const float4* acc4, int mode);
 
@@ -78,6 +80,7 @@ struct Engine {
     std::vector<SysDesc> hdesc;  // indexed by slot
     std::vector<Slot> active;
     bool any_bh = false, any_bf = false;
+    int bf_tb = 1;  // brute-force targets per lane of the current batch (1 or 2)
     int law = 1;
     bool kahan = false;
 

@@ -41,6 +41,17 @@ static void add_ranges(std::vector<BlkWork>& t, int sys, int count, int per) {
     for (int b = 0; b < count; b += per) t.push_back(BlkWork{sys, b, std::min(count, b + per), 0});
 }
 
+// Brute force: log2 of the waves that split one target group's sources, from N
+// alone (so a system sums in the same order in any batch): enough for 8192 waves
+// at one target per lane (two 16-wave workgroups per CU on 256 CUs), at most the
+// 16 waves of a workgroup.
+static int bf_split_log2(int n) {
+    const long long groups = (n + 63) / 64;
+    int lg = 0;
+    while (lg < 4 && (groups << lg) < 8192) ++lg;
+    return lg;
+}
+
 static int blocks_bound(long long items, int per, int systems) {
     return (int)std::min<long long>((items + per - 1) / per + 2LL * systems, 0x7fffffff);
 }
@@ -73,7 +84,7 @@ nb_status Engine::reserve(const Capacity& c, int law_, bool kahan_, void* user_s
     tab_cap[T_BODY] = blocks_bound(c.bodies, 256, S);
     tab_cap[T_CELLS] = blocks_bound(c.cells, 256, S);
     tab_cap[T_GROUPS] = blocks_bound(c.bodies, 256, S);
-    tab_cap[T_BRUTE] = blocks_bound(c.bodies, 256, S);
+    tab_cap[T_BRUTE] = blocks_bound(c.bodies, 64, S);
     tab_cap[T_UPDATE] = blocks_bound(c.bodies, 256, S);
     for (int ph = 0; ph < kBuildPhases; ++ph)
         tab_cap[T_PHASE0 + ph] = blocks_bound((c.bodies + kPhaseStride[ph] - 1) / kPhaseStride[ph], kBuildBlock, S);
@@ -127,6 +138,7 @@ nb_status Engine::activate(const std::vector<Slot>& slots) {
     std::vector<BlkWork> t[T_COUNT];
     std::vector<int> ids;
     std::vector<char> seen(capc.systems, 0);
+    long long bf_waves2 = 0;  // brute-force waves of the batch at two targets per lane
     any_bh = any_bf = false;
     for (const Slot& sl : slots) {
         if (sl.slot < 0 || sl.slot >= capc.systems || seen[sl.slot]) {
@@ -177,9 +189,17 @@ nb_status Engine::activate(const std::vector<Slot>& slots) {
         }
         if (!bh && d.n > 0) {
             any_bf = true;
-            add_ranges(t[T_BRUTE], s, d.n, 256);
+            d.bf_split_log2 = bf_split_log2(d.n);
+            bf_waves2 += (long long)((d.n + 127) / 128) << d.bf_split_log2;
         }
         add_ranges(t[T_UPDATE], s, d.n, 256);
+    }
+    // two targets per lane only when the batch still fills every CU twice over
+    bf_tb = bf_waves2 >= 8192 ? 2 : 1;
+    for (const Slot& sl : slots) {
+        const SysDesc& d = hdesc[sl.slot];
+        if (d.algo != NB_BARNES_HUT && d.n > 0)
+            add_ranges(t[T_BRUTE], sl.slot, d.n, (kBfWaves >> d.bf_split_log2) * 64 * bf_tb);
     }
     for (int k = 0; k < T_COUNT; ++k) {
         if ((int)t[k].size() > tab_cap[k]) {
@@ -263,7 +283,7 @@ void Engine::force() {
                     acc4);
 }
 void Engine::brute() {
-    launch_brute(stream, tab_len[T_BRUTE], law, kahan, d_desc, d_state, tab[T_BRUTE], node4, acc4);
+    launch_brute(stream, tab_len[T_BRUTE], law, kahan, bf_tb, d_desc, d_state, tab[T_BRUTE], node4, acc4);
 }
 void Engine::update(int mode) {
     launch_update(stream, tab_len[T_UPDATE], d_desc, d_state, tab[T_UPDATE], node4, vel4, acc4, mode);

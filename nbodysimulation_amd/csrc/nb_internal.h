@@ -23,6 +23,8 @@ namespace nb {
 
 constexpr int kMaxLevel = 64;          // deepest cell level (root = 0)
 constexpr int kDqLen = kMaxLevel + 1;  // opening thresholds per level
+constexpr int kBfWaves = 16;    // brute-force workgroup: 16 waves of 64
+constexpr int kEnergyLen = 9;   // kinetic, potential, p[3], L[3], mass
 constexpr int kEmpty = -1;
 constexpr int kLocked = -2;
 
@@ -46,7 +48,8 @@ struct SysDesc {
     float G, theta, eps, dt, ueps;
     int bbox_first;   // first global block of this system in the bounds table
     int bbox_blocks;  // number of bounds blocks of this system
-    int pad[3];
+    int bf_split_log2;  // brute force: log2 of the waves sharing one target group
+    int pad[2];
 };
 
 struct SysState {

@@ -637,58 +637,208 @@ __global__ __launch_bounds__(256) void k_force_bh(const SysDesc* __restrict__ sd
 }
 
 // ---------------------------------------------------------------------------
-// brute force: all pairs, one target per lane, j-bodies read wave-uniformly.
+// brute force: all pairs (brute_force_update.cl:3-105 `update`, :108-228 `update_local`).
 // LAW 0: d*m*rsqrt(|d|^2+eps^2)^3.  LAW 1 (brute_force_update.cl:54-58,114-120):
-// n = normalize(d), n*m/(|n|^2+ueps^2).  Self is skipped.
+// n = normalize(d), n*m/(|n|^2+ueps^2).
+//
+// Workgroup = 16 waves.  A system's targets are cut into groups of 64*TB (each
+// lane holds TB targets 64 apart: register blocking).  S = d.bf_split waves
+// share one group; wave `part` sums the sources [n*part/S, n*(part+1)/S) in
+// order, staged 64 at a time through its own LDS tile and read back as
+// broadcasts; the S partial sums are then added in wave order through LDS.
+// S depends on N only, and TB does not change any target's arithmetic, so a
+// system's result does not depend on the batch it runs in.
+// The self pair is not skipped: its displacement is exactly zero, so it adds
+// exactly zero (eps > 0 keeps the PLUMMER denominator finite; LAW 1 maps a
+// zero displacement to a zero unit vector).
 
-template <int LAW, bool KAHAN>
-__global__ __launch_bounds__(256) void k_brute(const SysDesc* __restrict__ sd, const SysState* __restrict__ st,
-                                               const BlkWork* __restrict__ work,
-                                               const float4* __restrict__ node4, float4* __restrict__ acc4) {
+// Order LDS accesses of one wave (same-wave LDS operations complete in order).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int LAW>
+__device__ __forceinline__ void bf_pair(const float4 q, float px, float py, float pz, float e2, float& fx, float& fy,
+                                        float& fz, float& s) {
+    const float dx = q.x - px, dy = q.y - py, dz = q.z - pz;
+    if (LAW == 0) {
+        const float d2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, e2)));
+        const float ri = __builtin_amdgcn_rsqf(d2);
+        const float ri2 = ri * ri;
+        const float ri3 = ri2 * ri;
+        s = q.w * ri3;
+        fx = dx; fy = dy; fz = dz;
+    } else {
+        const float l2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        const float ri = l2 > 0.0f ? __builtin_amdgcn_rsqf(l2) : 0.0f;
+        fx = dx * ri; fy = dy * ri; fz = dz * ri;
+        const float d2 = fmaf(fz, fz, fmaf(fy, fy, fmaf(fx, fx, e2)));
+        s = q.w * __builtin_amdgcn_rcpf(d2);
+    }
+}
+
+__device__ __forceinline__ void kahan_add(float& a, float& c, float v) {
+    const float y = v - c;
+    const float t = a + y;
+    c = (t - a) - y;
+    a = t;
+}
+
+template <int LAW, bool KAHAN, int TB>
+__global__ __launch_bounds__(1024) void k_brute(const SysDesc* __restrict__ sd, const SysState* __restrict__ st,
+                                                const BlkWork* __restrict__ work,
+                                                const float4* __restrict__ node4, float4* __restrict__ acc4) {
+    constexpr int NC = KAHAN ? 6 : 3;
+    __shared__ float4 tile[kBfWaves][64];
+    __shared__ float red[kBfWaves][NC * TB][64];
     const BlkWork w = work[blockIdx.x];
     const SysDesc d = sd[w.sys];
-    if (st[w.sys].status) return;
-    const int i = w.begin + threadIdx.x;
-    const bool valid = i < w.end;
-    const int ic = valid ? i : w.end - 1;
+    if (st[w.sys].status) return;  // whole workgroup
+    // the wave index is uniform over the wave: keep it (and the loop bounds) scalar
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int lg = d.bf_split_log2;
+    const int S = 1 << lg;
+    const int grp = wave >> lg, part = wave & (S - 1);
+    const int n = d.n;
     const float4* __restrict__ nodes = node4 + d.node_off;
-    const float4 p = nodes[ic];
     const float e = LAW == 0 ? d.eps : d.ueps;
     const float e2 = e * e;
-    float ax = 0.f, ay = 0.f, az = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
-    const int n = d.n;
-    for (int j = 0; j < n; ++j) {
-        const float4 q = nodes[j];
-        const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
-        float fx, fy, fz, s;
-        if (LAW == 0) {
-            const float d2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, e2)));
-            const float ri = __builtin_amdgcn_rsqf(d2);
-            s = q.w * (ri * ri * ri);
-            fx = dx; fy = dy; fz = dz;
-        } else {
-            const float l2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            const float ri = l2 > 0.0f ? __builtin_amdgcn_rsqf(l2) : 0.0f;
-            fx = dx * ri; fy = dy * ri; fz = dz * ri;
-            const float d2 = fmaf(fz, fz, fmaf(fy, fy, fmaf(fx, fx, e2)));
-            s = q.w * __builtin_amdgcn_rcpf(d2);
-        }
-        s = (j == ic) ? 0.0f : s;
-        if (KAHAN) {
-            float y, t;
-            y = fx * s - kx; t = ax + y; kx = (t - ax) - y; ax = t;
-            y = fy * s - ky; t = ay + y; ky = (t - ay) - y; ay = t;
-            y = fz * s - kz; t = az + y; kz = (t - az) - y; az = t;
-        } else {
-            ax = fmaf(fx, s, ax);
-            ay = fmaf(fy, s, ay);
-            az = fmaf(fz, s, az);
+    float px[TB], py[TB], pz[TB], acc[NC][TB];
+#pragma unroll
+    for (int t = 0; t < TB; ++t) {
+        const int i = min(w.begin + grp * 64 * TB + t * 64 + lane, w.end - 1);
+        const float4 p = nodes[i];
+        px[t] = p.x; py[t] = p.y; pz[t] = p.z;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[c][t] = 0.0f;
+    }
+    const int j0 = (int)((long long)n * part >> lg), j1 = (int)((long long)n * (part + 1) >> lg);
+    for (int jt = j0; jt < j1; jt += 64) {
+        const int cnt = min(64, j1 - jt);
+        wave_sync();
+        if (lane < cnt) tile[wave][lane] = nodes[jt + lane];
+        wave_sync();
+#pragma unroll 4
+        for (int k = 0; k < cnt; ++k) {
+            const float4 q = tile[wave][k];
+#pragma unroll
+            for (int t = 0; t < TB; ++t) {
+                float fx, fy, fz, sc;
+                bf_pair<LAW>(q, px[t], py[t], pz[t], e2, fx, fy, fz, sc);
+                if (KAHAN) {
+                    kahan_add(acc[0][t], acc[3][t], fx * sc);
+                    kahan_add(acc[1][t], acc[4][t], fy * sc);
+                    kahan_add(acc[2][t], acc[5][t], fz * sc);
+                } else {
+                    acc[0][t] = fmaf(fx, sc, acc[0][t]);
+                    acc[1][t] = fmaf(fy, sc, acc[1][t]);
+                    acc[2][t] = fmaf(fz, sc, acc[2][t]);
+                }
+            }
         }
     }
+    if (S > 1) {  // uniform over the workgroup
+        if (part != 0) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+#pragma unroll
+                for (int t = 0; t < TB; ++t) red[wave][c * TB + t][lane] = acc[c][t];
+        }
+        __syncthreads();
+        if (part != 0) return;
+        for (int q = 1; q < S; ++q) {
+#pragma unroll
+            for (int t = 0; t < TB; ++t) {
+                if (KAHAN) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        kahan_add(acc[c][t], acc[c + 3][t],
+                                  red[wave + q][c * TB + t][lane] - red[wave + q][(c + 3) * TB + t][lane]);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) acc[c][t] += red[wave + q][c * TB + t][lane];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < TB; ++t) {
+        const int i = w.begin + grp * 64 * TB + t * 64 + lane;
+        if (i < w.end) {
+            float ax = acc[0][t], ay = acc[1][t], az = acc[2][t];
+            if (KAHAN) { ax -= acc[3][t]; ay -= acc[4][t]; az -= acc[5][t]; }
+            acc4[d.body_off + i] = make_float4(ax * d.G, ay * d.G, az * d.G, (float)(n - 1));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 conservation diagnostics of one system (no counterpart in the reference).
+// Per body i: kinetic m v^2/2, momentum m v, angular momentum m x*v, mass, and
+// half its pair potential sum_j!=i phi(r_ij) (PLUMMER: -G m_i m_j/sqrt(r^2+eps^2);
+// REFERENCE_UNIT: +G m_i m_j r/(1+eps^2)), all in FP64.  Sources are staged
+// through LDS 256 at a time.  Workgroup sums go to `partial` in a fixed tree
+// order; k_energy_final adds them in block order, so the result is deterministic.
+
+__device__ __forceinline__ void block_sum_f64(double* v, double (*sh)[256], int nv) {
+    const int t = threadIdx.x;
+    for (int c = 0; c < nv; ++c) sh[c][t] = v[c];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w)
+            for (int c = 0; c < nv; ++c) sh[c][t] += sh[c][t + w];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_energy_f64(int n, const float4* __restrict__ node4,
+                                                    const float4* __restrict__ vel4, int law, double eps,
+                                                    double G, double* __restrict__ partial) {
+    __shared__ float4 tile[256];
+    __shared__ double sh[kEnergyLen][256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = i < n;
+    const float4 p = node4[valid ? i : 0];
+    const double px = p.x, py = p.y, pz = p.z;
+    const double e2 = eps * eps, inv_unit = 1.0 / (1.0 + e2);
+    double pot = 0.0;
+    for (int jt = 0; jt < n; jt += 256) {
+        __syncthreads();
+        if (jt + threadIdx.x < n) tile[threadIdx.x] = node4[jt + threadIdx.x];
+        __syncthreads();
+        const int cnt = min(256, n - jt);
+        for (int k = 0; k < cnt; ++k) {
+            const float4 q = tile[k];
+            const double dx = (double)q.x - px, dy = (double)q.y - py, dz = (double)q.z - pz;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const double phi = law == 0 ? -1.0 / sqrt(r2 + e2) : sqrt(r2) * inv_unit;
+            pot += (jt + k == i) ? 0.0 : (double)q.w * phi;
+        }
+    }
+    double v[kEnergyLen] = {};
     if (valid) {
-        if (KAHAN) { ax -= kx; ay -= ky; az -= kz; }
-        acc4[d.body_off + i] = make_float4(ax * d.G, ay * d.G, az * d.G, (float)(n - 1));
+        const float4 u = vel4[i];
+        const double m = p.w, vx = u.x, vy = u.y, vz = u.z;
+        v[0] = 0.5 * m * (vx * vx + vy * vy + vz * vz);
+        v[1] = 0.5 * G * m * pot;
+        v[2] = m * vx; v[3] = m * vy; v[4] = m * vz;
+        v[5] = m * (py * vz - pz * vy); v[6] = m * (pz * vx - px * vz); v[7] = m * (px * vy - py * vx);
+        v[8] = m;
     }
+    block_sum_f64(v, sh, kEnergyLen);
+    if (threadIdx.x < kEnergyLen) partial[(size_t)blockIdx.x * kEnergyLen + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void k_energy_final(int nblk, const double* __restrict__ partial,
+                                                      double* __restrict__ out) {
+    __shared__ double sh[kEnergyLen][256];
+    double v[kEnergyLen] = {};
+    for (int b = threadIdx.x; b < nblk; b += 256)
+        for (int c = 0; c < kEnergyLen; ++c) v[c] += partial[(size_t)b * kEnergyLen + c];
+    block_sum_f64(v, sh, kEnergyLen);
+    if (threadIdx.x < kEnergyLen) out[threadIdx.x] = sh[threadIdx.x][0];
 }
 
 // ---------------------------------------------------------------------------
@@ -760,12 +910,20 @@ void launch_force_bh(hipStream_t s, int nblk, bool kahan, const SysDesc* sd, con
     else
         NB_LAUNCH(k_force_bh<false>, nblk, 256, s, sd, st, work, node4, child, cdata, sorted, acc4);
 }
-void launch_brute(hipStream_t s, int nblk, int law, bool kahan, const SysDesc* sd, const SysState* st,
+void launch_brute(hipStream_t s, int nblk, int law, bool kahan, int tb, const SysDesc* sd, const SysState* st,
                   const BlkWork* work, const float4* node4, float4* acc4) {
-    if (law == 0 && !kahan) NB_LAUNCH((k_brute<0, false>), nblk, 256, s, sd, st, work, node4, acc4);
-    if (law == 0 && kahan) NB_LAUNCH((k_brute<0, true>), nblk, 256, s, sd, st, work, node4, acc4);
-    if (law == 1 && !kahan) NB_LAUNCH((k_brute<1, false>), nblk, 256, s, sd, st, work, node4, acc4);
-    if (law == 1 && kahan) NB_LAUNCH((k_brute<1, true>), nblk, 256, s, sd, st, work, node4, acc4);
+    const int B = kBfWaves * 64;
+#define NB_BF(L, K, T) \
+    if (law == L && kahan == K && tb == T) NB_LAUNCH((k_brute<L, K, T>), nblk, B, s, sd, st, work, node4, acc4)
+    NB_BF(0, false, 1); NB_BF(0, true, 1); NB_BF(1, false, 1); NB_BF(1, true, 1);
+    NB_BF(0, false, 2); NB_BF(0, true, 2); NB_BF(1, false, 2); NB_BF(1, true, 2);
+#undef NB_BF
+}
+void launch_energy(hipStream_t s, int n, const float4* node4, const float4* vel4, int law, double eps, double G,
+                   double* partial, double* out) {
+    const int nblk = (n + 255) / 256;
+    NB_LAUNCH(k_energy_f64, nblk, 256, s, n, node4, vel4, law, eps, G, partial);
+    NB_LAUNCH(k_energy_final, 1, 256, s, nblk, partial, out);
 }
 void launch_update(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
                    float4* node4, float4* vel4, const float4* acc4, int mode) {

@@ -154,6 +154,13 @@ class NBodySim:
         self._call("nb_force_interactions", C.byref(t))
         return t.value
 
+    def energy(self) -> dict:
+        """FP64 kinetic/potential energy, momentum, angular momentum and mass of the current state (GPU)."""
+        out = np.zeros(9, np.float64)
+        self._call("nb_sim_energy", out.ctypes.data_as(L._DP))
+        return {"kinetic": out[0], "potential": out[1], "total": out[0] + out[1], "momentum": out[2:5].copy(),
+                "angular": out[5:8].copy(), "mass": out[8]}
+
     def stage_times(self):
         ms = np.zeros(L.NUM_STAGES, np.float32)
         self._call("nb_stage_times", _fp(ms))

@@ -520,25 +520,34 @@ void orc_drift(int n, float dt, const float* vel, float* pos) {
 
 /* FP64 conservation diagnostics: out = {kinetic, potential, px, py, pz}. */
 void orc_energy_momentum_f64(int law, double eps, double G, int n, const float* pos, const float* vel,
-                             const float* mass, double* out5) {
-    double ke = 0, pe = 0, px = 0, py = 0, pz = 0;
+                             const float* mass, double* out) {
+    /* out: kinetic, potential, px, py, pz, Lx, Ly, Lz, total mass.
+     * PLUMMER pair potential -G m_i m_j / sqrt(r^2 + eps^2) (the force law's potential);
+     * REFERENCE_UNIT (constant-magnitude pull G m_j / (1 + eps^2) along the unit vector,
+     * brute_force_update.cl:43-62) has pair potential +G m_i m_j r / (1 + eps^2). */
+    double ke = 0, pe = 0, px = 0, py = 0, pz = 0, lx = 0, ly = 0, lz = 0, mt = 0;
     for (int i = 0; i < n; ++i) {
         double m = mass[i], vx = vel[3 * i], vy = vel[3 * i + 1], vz = vel[3 * i + 2];
+        double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
         ke += 0.5 * m * (vx * vx + vy * vy + vz * vz);
         px += m * vx; py += m * vy; pz += m * vz;
+        lx += m * (y * vz - z * vy); ly += m * (z * vx - x * vz); lz += m * (x * vy - y * vx);
+        mt += m;
     }
-    (void)law;
 #pragma omp parallel for reduction(+ : pe) schedule(dynamic, 16)
     for (int i = 0; i < n; ++i) {
         double s = 0;
         for (int j = i + 1; j < n; ++j) {
             double dx = (double)pos[3 * j] - pos[3 * i], dy = (double)pos[3 * j + 1] - pos[3 * i + 1],
                    dz = (double)pos[3 * j + 2] - pos[3 * i + 2];
-            s += (double)mass[j] / sqrt(dx * dx + dy * dy + dz * dz + eps * eps);
+            double r2 = dx * dx + dy * dy + dz * dz;
+            s += law == ORC_LAW_PLUMMER ? -(double)mass[j] / sqrt(r2 + eps * eps)
+                                        : (double)mass[j] * sqrt(r2) / (1.0 + eps * eps);
         }
-        pe -= G * mass[i] * s;
+        pe += G * mass[i] * s;
     }
-    out5[0] = ke; out5[1] = pe; out5[2] = px; out5[3] = py; out5[4] = pz;
+    out[0] = ke; out[1] = pe; out[2] = px; out[3] = py; out[4] = pz;
+    out[5] = lx; out[6] = ly; out[7] = lz; out[8] = mt;
 }
 
 /* Full Barnes-Hut Euler step sequence (CMyApp::Update, src/MyApp.cpp:583-598):

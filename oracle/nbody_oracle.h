@@ -71,8 +71,10 @@ void orc_direct_f64(int law, double eps, double G, int n, const float* pos, cons
 void orc_euler(int n, float dt, const float* acc, float* vel, float* pos);
 void orc_kick(int n, float h, const float* acc, float* vel);
 void orc_drift(int n, float dt, const float* vel, float* pos);
+/* FP64 conservation quantities; out[9] = kinetic, potential, px, py, pz, Lx, Ly, Lz, total mass.
+ * No counterpart in the reference (it computes no diagnostics). */
 void orc_energy_momentum_f64(int law, double eps, double G, int n, const float* pos, const float* vel,
-                             const float* mass, double* out5);
+                             const float* mass, double* out);
 int orc_bh_step(int rules, int nsteps, int n, int cap, float theta, float eps, float G, float dt,
                 int vote, int kahan, float* pos, float* vel, const float* mass, float* acc);
 int orc_threads(void);

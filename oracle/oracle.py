@@ -218,12 +218,12 @@ def drift(vel, pos, dt):
     return pos.reshape(-1, 3)
 
 
-def energy_momentum(pos, vel, mass, eps, G):
-    out = np.zeros(5, np.float64)
+def energy_momentum(pos, vel, mass, eps, G, law=LAW_PLUMMER):
+    """FP64 kinetic and potential energy, linear and angular momentum, total mass."""
+    out = np.zeros(9, np.float64)
     pos, vel, mass = f32(pos).reshape(-1), f32(vel).reshape(-1), f32(mass).reshape(-1)
-    load().orc_energy_momentum_f64(LAW_PLUMMER, eps, G, mass.size, fp(pos), fp(vel), fp(mass),
-                                   out.ctypes.data_as(_DP))
-    return {"kinetic": out[0], "potential": out[1], "momentum": out[2:5]}
+    load().orc_energy_momentum_f64(law, eps, G, mass.size, fp(pos), fp(vel), fp(mass), out.ctypes.data_as(_DP))
+    return {"kinetic": out[0], "potential": out[1], "momentum": out[2:5], "angular": out[5:8], "mass": out[8]}
 
 
 def bh_steps(pos, vel, mass, rules, nsteps, theta, eps, G, dt, vote=64, kahan=False, cap=None):
