@@ -64,11 +64,12 @@ enum { NB_ACCUM_PLAIN = 0, NB_ACCUM_KAHAN = 1 };
 
 #define NB_MAX_LEVEL 64   /* deepest cell level (calculate_forces.hcl:1 MAX_DEPTH) */
 #define NB_PATH_STRIDE 72 /* bytes per body in nb_export_bodies paths */
-#define NB_NUM_STAGES 8
+#define NB_NUM_STAGES 9
 
 /* Stage indices for nb_stage_times. */
 enum { NB_STAGE_BOUNDS = 0, NB_STAGE_RESET = 1, NB_STAGE_BUILD = 2, NB_STAGE_SUMMARIZE = 3,
-       NB_STAGE_SORT = 4, NB_STAGE_FORCE = 5, NB_STAGE_UPDATE = 6, NB_STAGE_BRUTE = 7 };
+       NB_STAGE_SORT = 4, NB_STAGE_FORCE = 5, NB_STAGE_UPDATE = 6, NB_STAGE_BRUTE = 7,
+       NB_STAGE_COLLISION = 8 };
 
 /* Stepping parameters: SimulationConfig + BarnesHutConfig fields that reach the
  * kernels (src/SimulationConfig.h:85-104, src/BarnesHutConfig.h:58-60), plus
@@ -86,7 +87,7 @@ typedef struct nb_config {
     float theta;               /* BarnesHutConfig::theta (1.5) */
     float epsilon;             /* BarnesHutConfig::epsilon (0.01) */
     float dt;                  /* time step (the reference derives it from frame time) */
-    int32_t collision;         /* SimulationConfig::collision; must be 0 (stage not built) */
+    int32_t collision;         /* SimulationConfig::collision: 1 runs nb_stage_collision after the update */
     int32_t use_graph;         /* nb_step replays a captured HIP graph (default 1) */
     int32_t time_stages;       /* record per-stage HIP events (default 0) */
     int32_t reserved[5];
@@ -143,6 +144,10 @@ nb_status nb_stage_force(nb_sim* sim);
 nb_status nb_stage_update(nb_sim* sim);
 /* update / update_local all-pairs force (kernels/brute_force_update.cl:3-228); writes acc. */
 nb_status nb_stage_brute_force(nb_sim* sim);
+/* update_collision_local (kernels/collision_update.cl:3-83): bodies are spheres of radius
+ * 0.015; each body's velocity is reflected about the separation of every other body within
+ * 0.03, in index order.  O(N^2); reads positions, writes velocities. */
+nb_status nb_stage_collision(nb_sim* sim);
 /* nsteps whole steps (BH: bounds..update; BF: brute force + update). */
 nb_status nb_step(nb_sim* sim, int32_t nsteps);
 
@@ -162,6 +167,31 @@ nb_status nb_stage_times(nb_sim* sim, float* ms);
 /* nsteps whole steps queued on the stream without waiting for them (no status
  * check; call nb_sim_sync / nb_sim_status afterwards). */
 nb_status nb_step_async(nb_sim* sim, int32_t nsteps);
+
+/* Vertex export, the equivalent of copy_vertices (kernels/copy_vertices.cl:3-17, which fills
+ * the GL vertex buffer): (x, y, z, 1) per body, float[4*N].  dst_is_device = 0: dst is host
+ * memory (synchronous); 1: dst is device memory, written on the simulation's stream. */
+nb_status nb_export_vertices(nb_sim* sim, float* dst_xyzw, int32_t dst_is_device);
+
+/* Checkpoints (no counterpart in the reference, which keeps no state across runs): a
+ * self-describing byte blob = nb_checkpoint_header, then float pos[3N], vel[3N], mass[N],
+ * acc[3N].  Loading needs a simulation of the same num_bodies; it restores the state, the
+ * accelerations (so LEAPFROG continues without a new force pass) and the step counter. */
+#define NB_CHECKPOINT_MAGIC 0x4b43424eu /* "NBCK" */
+#define NB_CHECKPOINT_VERSION 1
+typedef struct nb_checkpoint_header {
+    uint32_t magic;
+    uint32_t version;
+    int32_t num_bodies;
+    int32_t have_acc;   /* acc holds a(x) of the stored positions */
+    int64_t step;       /* steps taken since the state was first uploaded */
+    nb_config config;   /* the simulation's configuration when saved */
+} nb_checkpoint_header;
+nb_status nb_checkpoint_size(nb_sim* sim, size_t* bytes);
+nb_status nb_checkpoint_save(nb_sim* sim, void* buf, size_t len);
+nb_status nb_checkpoint_load(nb_sim* sim, const void* buf, size_t len);
+/* Steps taken since the last upload or checkpoint load (plus the loaded counter). */
+nb_status nb_sim_step_count(nb_sim* sim, int64_t* steps);
 
 /* FP64 conservation diagnostics of the current state, computed on the GPU (no
  * counterpart in the reference).  out[NB_ENERGY_LEN] = kinetic, potential,

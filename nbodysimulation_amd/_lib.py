@@ -36,8 +36,8 @@ IC_NORMAL, IC_PLUMMER, IC_UNIFORM_BALL, IC_COLLISION = 0, 1, 2, 3
 MAX_LEVEL = 64
 DQ_LEN = MAX_LEVEL + 1
 PATH_STRIDE = 72
-NUM_STAGES = 8
-STAGE_NAMES = ("bounds", "reset", "build", "summarize", "sort", "force", "update", "brute")
+NUM_STAGES = 9
+STAGE_NAMES = ("bounds", "reset", "build", "summarize", "sort", "force", "update", "brute", "collision")
 
 
 class NbConfig(C.Structure):
@@ -81,6 +81,21 @@ class NbSystemSpec(C.Structure):
         ("epsilon", C.c_float),
         ("dt", C.c_float),
     ]
+
+
+class NbCheckpointHeader(C.Structure):
+    _fields_ = [
+        ("magic", C.c_uint32),
+        ("version", C.c_uint32),
+        ("num_bodies", C.c_int32),
+        ("have_acc", C.c_int32),
+        ("step", C.c_int64),
+        ("config", NbConfig),
+    ]
+
+
+CHECKPOINT_MAGIC = 0x4B43424E
+CHECKPOINT_VERSION = 1
 
 
 class NbServerStats(C.Structure):
@@ -152,6 +167,7 @@ SIGNATURES = {
     "nb_stage_force": (C.c_int, [_P]),
     "nb_stage_update": (C.c_int, [_P]),
     "nb_stage_brute_force": (C.c_int, [_P]),
+    "nb_stage_collision": (C.c_int, [_P]),
     "nb_step": (C.c_int, [_P, C.c_int32]),
     "nb_tree_info_get": (C.c_int, [_P, C.POINTER(NbTreeInfo)]),
     "nb_export_bodies": (C.c_int, [_P, _IP, _U8P, _IP]),
@@ -161,6 +177,11 @@ SIGNATURES = {
     "nb_stage_times": (C.c_int, [_P, _FP]),
     "nb_step_async": (C.c_int, [_P, C.c_int32]),
     "nb_sim_energy": (C.c_int, [_P, _DP]),
+    "nb_export_vertices": (C.c_int, [_P, _FP, C.c_int32]),
+    "nb_sim_step_count": (C.c_int, [_P, _I64P]),
+    "nb_checkpoint_size": (C.c_int, [_P, C.POINTER(C.c_size_t)]),
+    "nb_checkpoint_save": (C.c_int, [_P, C.c_void_p, C.c_size_t]),
+    "nb_checkpoint_load": (C.c_int, [_P, C.c_void_p, C.c_size_t]),
     "nb_ens_create": (C.c_int, [C.POINTER(NbSystemSpec), C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P,
                                 C.POINTER(_P)]),
     "nb_ens_destroy": (C.c_int, [_P]),

@@ -32,9 +32,10 @@ struct nb_sim {
     bool have_acc = false;
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_def = nullptr;
-    std::vector<hipEvent_t> ev;  // 8 per timed step
+    std::vector<hipEvent_t> ev;  // kEvPerStep per timed step
     float stage_ms[NB_NUM_STAGES] = {};
     double* energy_buf = nullptr;  // per-workgroup partial sums + result of nb_sim_energy
+    int64_t steps = 0;             // steps since upload (or the loaded checkpoint's count)
 };
 
 #define NB_HIPC(call)                                                                  \
@@ -126,7 +127,7 @@ static nb_status validate(const nb_config* c) {
     if (c->law != NB_LAW_PLUMMER && c->law != NB_LAW_REFERENCE_UNIT) return fail(NB_E_INVALID_ARG, "bad law");
     if (c->accumulation != NB_ACCUM_PLAIN && c->accumulation != NB_ACCUM_KAHAN)
         return fail(NB_E_INVALID_ARG, "bad accumulation");
-    if (c->collision) return fail(NB_E_INVALID_ARG, "collision stage is not part of this engine");
+    if (c->collision != 0 && c->collision != 1) return fail(NB_E_INVALID_ARG, "collision must be 0 or 1");
     if (!std::isfinite(c->dt) || !std::isfinite(c->gravitational_constant) || !std::isfinite(c->theta))
         return fail(NB_E_INVALID_ARG, "non-finite dt, G or theta");
     const bool needs_eps = c->algorithm == NB_BARNES_HUT || c->law == NB_LAW_PLUMMER;
@@ -216,6 +217,7 @@ nb_status nb_sim_upload(nb_sim* sim, const float* pos, const float* vel, const f
     NB_HIPC(hipStreamSynchronize(e.stream));
     sim->have_state = true;
     sim->have_box = sim->have_tree = sim->have_summary = sim->have_sorted = sim->have_acc = false;
+    sim->steps = 0;
     return NB_OK;
 }
 
@@ -349,12 +351,21 @@ nb_status nb_stage_brute_force(nb_sim* sim) {
     return NB_OK;
 }
 
+nb_status nb_stage_collision(nb_sim* sim) {
+    if (!sim) return fail(NB_E_INVALID_ARG, "NULL sim");
+    nb_status st = need(sim->have_state, "collision needs uploaded state");
+    if (st) return st;
+    sim->eng.collide();
+    return NB_OK;
+}
+
 nb_status nb_stage_update(nb_sim* sim) {
     if (!sim) return fail(NB_E_INVALID_ARG, "NULL sim");
     nb_status st = need(sim->have_acc, "update needs accelerations");
     if (st) return st;
     sim->eng.update(sim->cfg.method == NB_EULER ? 0 : 1);
     sim->have_box = false;
+    ++sim->steps;
     return NB_OK;
 }
 
@@ -393,10 +404,14 @@ static void enqueue_step(nb_sim* sim, hipEvent_t* e) {
     enqueue_force(sim, e);
     g.update(lf ? 1 : 0);
     if (e) (void)hipEventRecord(e[7], g.stream);
+    if (sim->cfg.collision) g.collide();  // after the integrator, as CMyApp::Update orders it
+    if (e) (void)hipEventRecord(e[8], g.stream);
 }
 
+static constexpr size_t kEvPerStep = 9;
+
 static nb_status ensure_events(nb_sim* sim, size_t nsteps) {
-    while (sim->ev.size() < 8 * nsteps) {
+    while (sim->ev.size() < kEvPerStep * nsteps) {
         hipEvent_t e = nullptr;
         NB_HIPC(hipEventCreate(&e));
         sim->ev.push_back(e);
@@ -408,8 +423,8 @@ static nb_status ensure_events(nb_sim* sim, size_t nsteps) {
 static nb_status read_stage_times(nb_sim* sim, int nsteps) {
     std::fill(sim->stage_ms, sim->stage_ms + NB_NUM_STAGES, 0.0f);
     for (int s = 0; s < nsteps; ++s) {
-        hipEvent_t* e = sim->ev.data() + 8 * (size_t)s;
-        NB_HIPC(hipEventSynchronize(e[7]));
+        hipEvent_t* e = sim->ev.data() + kEvPerStep * (size_t)s;
+        NB_HIPC(hipEventSynchronize(e[8]));
         float ms = 0.0f;
         if (is_bh(sim)) {
             for (int i = 0; i < 7; ++i) {
@@ -422,6 +437,8 @@ static nb_status read_stage_times(nb_sim* sim, int nsteps) {
             NB_HIPC(hipEventElapsedTime(&ms, e[6], e[7]));
             sim->stage_ms[NB_STAGE_UPDATE] += ms;
         }
+        NB_HIPC(hipEventElapsedTime(&ms, e[7], e[8]));
+        sim->stage_ms[NB_STAGE_COLLISION] += ms;
     }
     return NB_OK;
 }
@@ -466,7 +483,7 @@ nb_status nb_step(nb_sim* sim, int32_t nsteps) {
         }
         for (; done < nsteps; ++done) NB_HIPC(hipGraphLaunch(sim->graph, e.stream));
     } else {
-        for (; done < nsteps; ++done) enqueue_step(sim, timed ? sim->ev.data() + 8 * (size_t)done : nullptr);
+        for (; done < nsteps; ++done) enqueue_step(sim, timed ? sim->ev.data() + kEvPerStep * (size_t)done : nullptr);
     }
     NB_HIPC(hipGetLastError());
     NB_HIPC(hipStreamSynchronize(e.stream));
@@ -478,6 +495,7 @@ nb_status nb_step(nb_sim* sim, int32_t nsteps) {
     sim->tree_clean = false;
     sim->have_tree = sim->have_summary = sim->have_sorted = is_bh(sim);
     sim->have_acc = true;
+    sim->steps += nsteps;
     return nb_sim_status(sim);
 }
 
@@ -504,6 +522,7 @@ nb_status nb_step_async(nb_sim* sim, int32_t nsteps) {
     sim->tree_clean = false;
     sim->have_tree = sim->have_summary = sim->have_sorted = is_bh(sim);
     sim->have_acc = true;
+    sim->steps += nsteps;
     return NB_OK;
 }
 
@@ -527,6 +546,97 @@ nb_status nb_sim_energy(nb_sim* sim, double* out) {
     NB_HIPC(hipGetLastError());
     NB_HIPC(hipMemcpyAsync(out, res, sizeof(double) * nb::kEnergyLen, hipMemcpyDeviceToHost, g.stream));
     NB_HIPC(hipStreamSynchronize(g.stream));
+    return NB_OK;
+}
+
+nb_status nb_export_vertices(nb_sim* sim, float* dst, int32_t dst_is_device) {
+    if (!sim || (!dst && sim->n > 0)) return fail(NB_E_INVALID_ARG, "NULL argument");
+    if (!sim->have_state) return fail(NB_E_STATE, "no state uploaded");
+    Engine& g = sim->eng;
+    const int n = sim->n;
+    if (n == 0) return NB_OK;
+    if (dst_is_device) {
+        nb::launch_vertices(g.stream, n, g.node4, reinterpret_cast<float4*>(dst));
+        NB_HIPC(hipGetLastError());
+        return NB_OK;
+    }
+    NB_HIPC(hipMemcpyAsync(dst, g.node4, sizeof(float4) * n, hipMemcpyDeviceToHost, g.stream));
+    NB_HIPC(hipStreamSynchronize(g.stream));
+    for (int i = 0; i < n; ++i) dst[4 * i + 3] = 1.0f;
+    return NB_OK;
+}
+
+nb_status nb_sim_step_count(nb_sim* sim, int64_t* steps) {
+    if (!sim || !steps) return fail(NB_E_INVALID_ARG, "NULL argument");
+    *steps = sim->steps;
+    return NB_OK;
+}
+
+static size_t checkpoint_bytes(int n) { return sizeof(nb_checkpoint_header) + sizeof(float) * 10 * (size_t)n; }
+
+nb_status nb_checkpoint_size(nb_sim* sim, size_t* bytes) {
+    if (!sim || !bytes) return fail(NB_E_INVALID_ARG, "NULL argument");
+    *bytes = checkpoint_bytes(sim->n);
+    return NB_OK;
+}
+
+nb_status nb_checkpoint_save(nb_sim* sim, void* buf, size_t len) {
+    if (!sim || !buf) return fail(NB_E_INVALID_ARG, "NULL argument");
+    if (!sim->have_state) return fail(NB_E_STATE, "no state uploaded");
+    const int n = sim->n;
+    if (len < checkpoint_bytes(n)) return fail(NB_E_INVALID_ARG, "buffer smaller than nb_checkpoint_size");
+    NB_HIPC(hipStreamSynchronize(sim->eng.stream));
+    nb_checkpoint_header h;
+    std::memset(&h, 0, sizeof(h));
+    h.magic = NB_CHECKPOINT_MAGIC;
+    h.version = NB_CHECKPOINT_VERSION;
+    h.num_bodies = n;
+    h.have_acc = sim->have_acc ? 1 : 0;
+    h.step = sim->steps;
+    h.config = sim->cfg;
+    char* out = static_cast<char*>(buf);
+    std::memcpy(out, &h, sizeof(h));
+    float* pos = reinterpret_cast<float*>(out + sizeof(h));
+    float* vel = pos + 3 * (size_t)n;
+    float* mass = vel + 3 * (size_t)n;
+    float* acc = mass + (size_t)n;
+    nb_status st = nb_sim_download(sim, pos, vel, acc);
+    if (st) return st;
+    if (n > 0) {
+        std::vector<float4> h4(n);
+        Engine& g = sim->eng;
+        NB_HIPC(hipMemcpyAsync(h4.data(), g.node4, sizeof(float4) * n, hipMemcpyDeviceToHost, g.stream));
+        NB_HIPC(hipStreamSynchronize(g.stream));
+        for (int i = 0; i < n; ++i) mass[i] = h4[i].w;
+    }
+    return NB_OK;
+}
+
+nb_status nb_checkpoint_load(nb_sim* sim, const void* buf, size_t len) {
+    if (!sim || !buf) return fail(NB_E_INVALID_ARG, "NULL argument");
+    if (len < sizeof(nb_checkpoint_header)) return fail(NB_E_INVALID_ARG, "truncated checkpoint");
+    nb_checkpoint_header h;
+    std::memcpy(&h, buf, sizeof(h));
+    if (h.magic != NB_CHECKPOINT_MAGIC || h.version != NB_CHECKPOINT_VERSION)
+        return fail(NB_E_INVALID_ARG, "not a checkpoint of this version");
+    if (h.num_bodies != sim->n) return fail(NB_E_INVALID_ARG, "checkpoint holds a different number of bodies");
+    const int n = sim->n;
+    if (len < checkpoint_bytes(n)) return fail(NB_E_INVALID_ARG, "truncated checkpoint");
+    const float* pos = reinterpret_cast<const float*>(static_cast<const char*>(buf) + sizeof(h));
+    const float* vel = pos + 3 * (size_t)n;
+    const float* mass = vel + 3 * (size_t)n;
+    const float* acc = mass + (size_t)n;
+    nb_status st = nb_sim_upload(sim, pos, vel, mass);
+    if (st) return st;
+    if (n > 0 && h.have_acc) {
+        std::vector<float4> a4(n);
+        for (int i = 0; i < n; ++i) a4[i] = make_float4(acc[3 * i], acc[3 * i + 1], acc[3 * i + 2], 0.0f);
+        Engine& g = sim->eng;
+        NB_HIPC(hipMemcpyAsync(g.acc4, a4.data(), sizeof(float4) * n, hipMemcpyHostToDevice, g.stream));
+        NB_HIPC(hipStreamSynchronize(g.stream));
+    }
+    sim->have_acc = h.have_acc != 0;
+    sim->steps = h.step;
     return NB_OK;
 }
 

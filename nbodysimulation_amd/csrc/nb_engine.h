@@ -34,6 +34,9 @@ void launch_force_bh(hipStream_t s, int nblk, bool kahan, const SysDesc* sd, con
                      const int* sorted, float4* acc4);
 void launch_brute(hipStream_t s, int nblk, int law, bool kahan, int tb, const SysDesc* sd, const SysState* st,
                   const BlkWork* work, const float4* node4, float4* acc4);
+void launch_collision(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
+                      const float4* node4, float4* vel4);
+void launch_vertices(hipStream_t s, int n, const float4* node4, float4* out);
 void launch_energy(hipStream_t s, int n, const float4* node4, const float4* vel4, int law, double eps, double G,
                    double* partial, double* out);
 void launch_update(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
@@ -122,6 +125,7 @@ struct Engine {
     void force();
     void brute();
     void update(int mode);
+    void collide();
 
     nb_status download_state(std::vector<SysState>& out);
 };

@@ -83,7 +83,7 @@ nb_status Engine::reserve(const Capacity& c, int law_, bool kahan_, void* user_s
     tab_cap[T_BOUNDS] = blocks_bound(c.bodies, kBoundsPerBlock, S);
     tab_cap[T_BODY] = blocks_bound(c.bodies, 256, S);
     tab_cap[T_CELLS] = blocks_bound(c.cells, 256, S);
-    tab_cap[T_GROUPS] = blocks_bound(c.bodies, 256, S);
+    tab_cap[T_GROUPS] = blocks_bound(c.bodies, 64 * kForceWaves, S);
     tab_cap[T_BRUTE] = blocks_bound(c.bodies, 64, S);
     tab_cap[T_UPDATE] = blocks_bound(c.bodies, 256, S);
     for (int ph = 0; ph < kBuildPhases; ++ph)
@@ -183,7 +183,7 @@ nb_status Engine::activate(const std::vector<Slot>& slots) {
             d.bbox_blocks = (int)t[T_BOUNDS].size() - d.bbox_first;
             add_ranges(t[T_BODY], s, d.n, 256);
             add_ranges(t[T_CELLS], s, d.cap, 256);
-            add_ranges(t[T_GROUPS], s, d.n, 256);
+            add_ranges(t[T_GROUPS], s, d.n, 64 * kForceWaves);
             for (int ph = 0; ph < kBuildPhases; ++ph)
                 add_ranges(t[T_PHASE0 + ph], s, (d.n + kPhaseStride[ph] - 1) / kPhaseStride[ph], kBuildBlock);
         }
@@ -287,6 +287,10 @@ void Engine::brute() {
 }
 void Engine::update(int mode) {
     launch_update(stream, tab_len[T_UPDATE], d_desc, d_state, tab[T_UPDATE], node4, vel4, acc4, mode);
+}
+
+void Engine::collide() {
+    launch_collision(stream, tab_len[T_UPDATE], d_desc, d_state, tab[T_UPDATE], node4, vel4);
 }
 
 nb_status Engine::download_state(std::vector<SysState>& out) {

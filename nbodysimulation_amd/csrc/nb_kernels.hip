@@ -10,6 +10,7 @@
 //   k_force_bh    calculate_force_local traversal            kernels/calculate_forces.cl:44-148
 //   k_brute       update / update_local all-pairs            kernels/brute_force_update.cl:3-228
 //   k_update      integrator branch                          kernels/calculate_forces.cl:150-182
+//   k_collision   update_collision_local                     kernels/collision_update.cl:3-83
 //
 // Memory-model notes (gfx950 has one L2 per XCD):
 //   * Data that one workgroup writes and another reads inside the same kernel is
@@ -555,12 +556,12 @@ __global__ __launch_bounds__(256) void k_sort(const SysDesc* __restrict__ sd, Sy
 constexpr int kStack = 8 * kDqLen;
 
 template <bool KAHAN>
-__global__ __launch_bounds__(256) void k_force_bh(const SysDesc* __restrict__ sd, const SysState* __restrict__ st,
+__global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __restrict__ sd, const SysState* __restrict__ st,
                                                   const BlkWork* __restrict__ work,
                                                   const float4* __restrict__ node4, const int* __restrict__ child,
                                                   const float4* __restrict__ cdata, const int* __restrict__ sorted,
                                                   float4* __restrict__ acc4) {
-    __shared__ int stk[4][kStack];
+    __shared__ int stk[kForceWaves][kStack];
     __shared__ float dqs[kDqLen];
     const BlkWork w = work[blockIdx.x];
     const SysDesc d = sd[w.sys];
@@ -775,6 +776,67 @@ __global__ __launch_bounds__(1024) void k_brute(const SysDesc* __restrict__ sd, 
 }
 
 // ---------------------------------------------------------------------------
+// collision (update_collision_local, collision_update.cl:17-83, with
+// calc_collision_traj :3-15): every body is a sphere of radius 0.015; for each
+// other body j in index order, if |p_j - p_i| <= r_i + r_j the velocity of i is
+// reflected about the unit separation n: v -= 2 (n.v) n, using the velocity as
+// already reflected by earlier j.  Positions are read-only, each lane writes
+// only its own velocity.  The reference tile loop re-reads tile 0 and skips the
+// wrong partner (the brute-force tile-loop quirk); this is the intended
+// all-pairs sweep.  A coincident pair has no separation direction and leaves v
+// unchanged.  Sources are staged 64 at a time through a per-wave LDS tile.
+
+constexpr float kCollisionRadius = 0.015f;
+
+__global__ __launch_bounds__(256) void k_collision(const SysDesc* __restrict__ sd, const SysState* __restrict__ st,
+                                                   const BlkWork* __restrict__ work,
+                                                   const float4* __restrict__ node4, float4* __restrict__ vel4) {
+    __shared__ float4 tile[4][64];
+    const BlkWork w = work[blockIdx.x];
+    const SysDesc d = sd[w.sys];
+    if (st[w.sys].status) return;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int i = min(w.begin + (int)threadIdx.x, w.end - 1);
+    const float4* __restrict__ nodes = node4 + d.node_off;
+    const float4 p = nodes[i];
+    float4 v = vel4[d.body_off + i];
+    const float rsum = kCollisionRadius + kCollisionRadius;
+    // l2 above this bound has sqrt(l2) > rsum for certain: the exact test is skipped
+    const float l2_skip = rsum * rsum * 1.0001f;
+    const int n = d.n;
+    for (int jt = 0; jt < n; jt += 64) {
+        const int cnt = min(64, n - jt);
+        wave_sync();
+        if (lane < cnt) tile[wave][lane] = nodes[jt + lane];
+        wave_sync();
+        for (int k = 0; k < cnt; ++k) {
+            const float4 q = tile[wave][k];
+            const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+            const float l2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            if (l2 > l2_skip || jt + k == i || l2 == 0.0f) continue;
+            const float dist = sqrtf(l2);
+            if (dist > rsum) continue;
+            const float nx = dx / dist, ny = dy / dist, nz = dz / dist;
+            const float k2 = 2.0f * fmaf(nz, v.z, fmaf(ny, v.y, nx * v.x));
+            v.x = fmaf(-k2, nx, v.x);
+            v.y = fmaf(-k2, ny, v.y);
+            v.z = fmaf(-k2, nz, v.z);
+        }
+    }
+    if (w.begin + (int)threadIdx.x < w.end) vel4[d.body_off + i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// vertex export (copy_vertices, copy_vertices.cl:3-17): (x, y, z, 1) per body.
+
+__global__ __launch_bounds__(256) void k_vertices(int n, const float4* __restrict__ node4, float4* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = node4[i];
+    out[i] = make_float4(p.x, p.y, p.z, 1.0f);
+}
+
+// ---------------------------------------------------------------------------
 // FP64 conservation diagnostics of one system (no counterpart in the reference).
 // Per body i: kinetic m v^2/2, momentum m v, angular momentum m x*v, mass, and
 // half its pair potential sum_j!=i phi(r_ij) (PLUMMER: -G m_i m_j/sqrt(r^2+eps^2);
@@ -906,9 +968,9 @@ void launch_force_bh(hipStream_t s, int nblk, bool kahan, const SysDesc* sd, con
                      const BlkWork* work, const float4* node4, const int* child, const float4* cdata,
                      const int* sorted, float4* acc4) {
     if (kahan)
-        NB_LAUNCH(k_force_bh<true>, nblk, 256, s, sd, st, work, node4, child, cdata, sorted, acc4);
+        NB_LAUNCH(k_force_bh<true>, nblk, 64 * kForceWaves, s, sd, st, work, node4, child, cdata, sorted, acc4);
     else
-        NB_LAUNCH(k_force_bh<false>, nblk, 256, s, sd, st, work, node4, child, cdata, sorted, acc4);
+        NB_LAUNCH(k_force_bh<false>, nblk, 64 * kForceWaves, s, sd, st, work, node4, child, cdata, sorted, acc4);
 }
 void launch_brute(hipStream_t s, int nblk, int law, bool kahan, int tb, const SysDesc* sd, const SysState* st,
                   const BlkWork* work, const float4* node4, float4* acc4) {
@@ -918,6 +980,13 @@ void launch_brute(hipStream_t s, int nblk, int law, bool kahan, int tb, const Sy
     NB_BF(0, false, 1); NB_BF(0, true, 1); NB_BF(1, false, 1); NB_BF(1, true, 1);
     NB_BF(0, false, 2); NB_BF(0, true, 2); NB_BF(1, false, 2); NB_BF(1, true, 2);
 #undef NB_BF
+}
+void launch_collision(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
+                      const float4* node4, float4* vel4) {
+    NB_LAUNCH(k_collision, nblk, 256, s, sd, st, work, node4, vel4);
+}
+void launch_vertices(hipStream_t s, int n, const float4* node4, float4* out) {
+    NB_LAUNCH(k_vertices, (n + 255) / 256, 256, s, n, node4, out);
 }
 void launch_energy(hipStream_t s, int n, const float4* node4, const float4* vel4, int law, double eps, double G,
                    double* partial, double* out) {

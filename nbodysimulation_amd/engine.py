@@ -25,14 +25,14 @@ def _ip(a):
 def make_config(n: int, algorithm: int = L.BARNES_HUT, *, method: int = L.EULER, rules: int = L.RULES_REFERENCE,
                 law: int = L.LAW_REFERENCE_UNIT, accumulation: int = L.ACCUM_PLAIN, G: float = 1.0,
                 theta: float = 0.5, eps: float = 0.01, dt: float = 1e-3, cell_capacity: int = 0,
-                use_graph: bool = True, time_stages: bool = False) -> L.NbConfig:
+                use_graph: bool = True, time_stages: bool = False, collision: bool = False) -> L.NbConfig:
     lib = L.load()
     c = L.NbConfig()
     lib.nb_config_default(c)
     c.num_bodies, c.algorithm, c.method, c.rules = n, algorithm, method, rules
     c.law, c.accumulation, c.gravitational_constant = law, accumulation, G
     c.theta, c.epsilon, c.dt, c.cell_capacity = theta, eps, dt, cell_capacity
-    c.use_graph, c.time_stages = int(use_graph), int(time_stages)
+    c.use_graph, c.time_stages, c.collision = int(use_graph), int(time_stages), int(collision)
     return c
 
 
@@ -103,6 +103,7 @@ class NBodySim:
     def sort(self): self._call("nb_stage_sort")
     def force(self): self._call("nb_stage_force")
     def brute_force(self): self._call("nb_stage_brute_force")
+    def collision(self): self._call("nb_stage_collision")
     def update(self): self._call("nb_stage_update")
 
     def force_pipeline(self):
@@ -160,6 +161,45 @@ class NBodySim:
         self._call("nb_sim_energy", out.ctypes.data_as(L._DP))
         return {"kinetic": out[0], "potential": out[1], "total": out[0] + out[1], "momentum": out[2:5].copy(),
                 "angular": out[5:8].copy(), "mass": out[8]}
+
+    def export_vertices(self) -> np.ndarray:
+        """(x, y, z, 1) per body, as the reference's copy_vertices kernel writes for rendering."""
+        out = np.empty(4 * self.n, np.float32)
+        self._call("nb_export_vertices", _fp(out), 0)
+        return out.reshape(-1, 4)
+
+    def step_count(self) -> int:
+        k = C.c_int64()
+        self._call("nb_sim_step_count", C.byref(k))
+        return k.value
+
+    def checkpoint(self) -> bytes:
+        """The state (positions, velocities, masses, accelerations, step counter, config) as bytes."""
+        size = C.c_size_t()
+        self._call("nb_checkpoint_size", C.byref(size))
+        buf = C.create_string_buffer(size.value)
+        self._call("nb_checkpoint_save", buf, size)
+        return buf.raw
+
+    def restore(self, blob: bytes):
+        buf = C.create_string_buffer(bytes(blob), len(blob))
+        self._call("nb_checkpoint_load", buf, len(blob))
+
+    def save(self, path: str):
+        with open(path, "wb") as f:
+            f.write(self.checkpoint())
+
+    @classmethod
+    def from_checkpoint(cls, path: str, stream: int | None = None) -> "NBodySim":
+        """A new simulation with the configuration and state stored in a checkpoint file."""
+        with open(path, "rb") as f:
+            blob = f.read()
+        h = L.NbCheckpointHeader.from_buffer_copy(blob[:C.sizeof(L.NbCheckpointHeader)])
+        if h.magic != L.CHECKPOINT_MAGIC or h.version != L.CHECKPOINT_VERSION:
+            raise ValueError(f"{path}: not a checkpoint of version {L.CHECKPOINT_VERSION}")
+        sim = cls(h.config, stream=stream)
+        sim.restore(blob)
+        return sim
 
     def stage_times(self):
         ms = np.zeros(L.NUM_STAGES, np.float32)

@@ -519,6 +519,35 @@ void orc_drift(int n, float dt, const float* vel, float* pos) {
 }
 
 /* FP64 conservation diagnostics: out = {kinetic, potential, px, py, pz}. */
+/* Collision stage (update_collision_local + calc_collision_traj, kernels/collision_update.cl:3-83),
+ * as intended: spheres of radius 0.015; for each body i and every other body j in index order,
+ * if distance(p_i, p_j) <= r_i + r_j then v_i -= 2 (n.v_i) n with n = (p_j - p_i)/|p_j - p_i|,
+ * using v_i as already reflected by earlier j (collision_update.cl:62-64).  The reference tile loop
+ * (:48-70) re-reads tile 0 and tests the wrong partner for self; this sweeps all pairs once.
+ * A coincident pair has no direction and leaves v_i unchanged. */
+void orc_collision(int n, const float* pos, float* vel) {
+    const float r = 0.015f, rsum = r + r;
+    float* out = (float*)malloc(sizeof(float) * 3 * (size_t)(n > 0 ? n : 1));
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < n; ++i) {
+        float vx = vel[3 * i], vy = vel[3 * i + 1], vz = vel[3 * i + 2];
+        for (int j = 0; j < n; ++j) {
+            if (j == i) continue;
+            float dx = pos[3 * j] - pos[3 * i], dy = pos[3 * j + 1] - pos[3 * i + 1], dz = pos[3 * j + 2] - pos[3 * i + 2];
+            float l2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            if (l2 == 0.0f) continue;
+            float dist = sqrtf(l2);
+            if (dist > rsum) continue;
+            float nx = dx / dist, ny = dy / dist, nz = dz / dist;
+            float k2 = 2.0f * fmaf(nz, vz, fmaf(ny, vy, nx * vx));
+            vx = fmaf(-k2, nx, vx); vy = fmaf(-k2, ny, vy); vz = fmaf(-k2, nz, vz);
+        }
+        out[3 * i] = vx; out[3 * i + 1] = vy; out[3 * i + 2] = vz;
+    }
+    memcpy(vel, out, sizeof(float) * 3 * (size_t)n);
+    free(out);
+}
+
 void orc_energy_momentum_f64(int law, double eps, double G, int n, const float* pos, const float* vel,
                              const float* mass, double* out) {
     /* out: kinetic, potential, px, py, pz, Lx, Ly, Lz, total mass.

@@ -71,6 +71,8 @@ void orc_direct_f64(int law, double eps, double G, int n, const float* pos, cons
 void orc_euler(int n, float dt, const float* acc, float* vel, float* pos);
 void orc_kick(int n, float h, const float* acc, float* vel);
 void orc_drift(int n, float dt, const float* vel, float* pos);
+/* Collision stage: reflects velocities of bodies closer than 2 x 0.015 (in place). */
+void orc_collision(int n, const float* pos, float* vel);
 /* FP64 conservation quantities; out[9] = kinetic, potential, px, py, pz, Lx, Ly, Lz, total mass.
  * No counterpart in the reference (it computes no diagnostics). */
 void orc_energy_momentum_f64(int law, double eps, double G, int n, const float* pos, const float* vel,

@@ -50,6 +50,7 @@ _SIG = {
     "orc_euler": (None, [C.c_int, C.c_float, _FP, _FP, _FP]),
     "orc_kick": (None, [C.c_int, C.c_float, _FP, _FP]),
     "orc_drift": (None, [C.c_int, C.c_float, _FP, _FP]),
+    "orc_collision": (None, [C.c_int, _FP, _FP]),
     "orc_energy_momentum_f64": (None, [C.c_int, C.c_double, C.c_double, C.c_int, _FP, _FP, _FP, _DP]),
     "orc_bh_step": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
                               C.c_int, C.c_int, _FP, _FP, _FP, _FP]),
@@ -216,6 +217,14 @@ def drift(vel, pos, dt):
     vel, pos = f32(vel).reshape(-1), f32(pos).reshape(-1).copy()
     load().orc_drift(pos.size // 3, dt, fp(vel), fp(pos))
     return pos.reshape(-1, 3)
+
+
+def collision(pos, vel):
+    """Velocities after the collision stage (reflection of pairs closer than 0.03)."""
+    pos = f32(pos).reshape(-1)
+    vel = f32(vel).reshape(-1).copy()
+    load().orc_collision(pos.size // 3, fp(pos), fp(vel))
+    return vel.reshape(-1, 3)
 
 
 def energy_momentum(pos, vel, mass, eps, G, law=LAW_PLUMMER):

@@ -124,7 +124,7 @@ def test_invalid_configs_rejected_before_device(nbl):
         c = make_config(n, L.BARNES_HUT, **kw)
         assert nbl.nb_sim_create(C.byref(c), None, C.byref(h)) == L.NB_E_INVALID_ARG
     c = make_config(10, L.BARNES_HUT)
-    c.collision = 1
+    c.collision = 2
     assert nbl.nb_sim_create(C.byref(c), None, C.byref(h)) == L.NB_E_INVALID_ARG
 
 

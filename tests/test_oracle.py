@@ -435,3 +435,61 @@ def test_energy_invariants_under_translation(orc):
     b = orc.energy_momentum(pos + np.float32(0.25), vel, mass, 0.01, 1.0)
     assert b["potential"] == pytest.approx(a["potential"], rel=1e-6)
     assert b["kinetic"] == a["kinetic"] and np.array_equal(a["momentum"], b["momentum"])
+
+
+def test_collision_two_body_reflection(orc):
+    """Head-on pair at 0.02 < 2 x 0.015: each velocity is reflected about the separation."""
+    pos = np.array([[0, 0, 0], [0.02, 0, 0]], np.float32)
+    vel = np.array([[1.0, 0.5, 0.0], [-1.0, 0.0, 0.25]], np.float32)
+    out = orc.collision(pos, vel)
+    assert np.array_equal(out[0], np.array([-1.0, 0.5, 0.0], np.float32))
+    assert np.array_equal(out[1], np.array([1.0, 0.0, 0.25], np.float32))
+    far = pos.copy()
+    far[1, 0] = 0.031
+    assert np.array_equal(orc.collision(far, vel), vel)
+
+
+def test_collision_sequential_order_and_speed(orc):
+    """Reflections apply in index order (they do not commute) and each preserves |v|."""
+    s = np.float32(0.6)
+    c = np.float32(0.8)
+    pos = np.array([[0, 0, 0], [0.02, 0, 0], [0.02 * c, 0.02 * s, 0]], np.float32)
+    vel = np.array([[0.3, -0.7, 0.2], [0, 0, 0], [0, 0, 0]], np.float32)
+    out = orc.collision(pos, vel)
+    v = vel[0].astype(np.float64)
+    for j in (1, 2):  # float64 restatement in index order
+        d = pos[j].astype(np.float64) - pos[0]
+        nrm = d / np.linalg.norm(d)
+        v = v - 2 * nrm.dot(v) * nrm
+    assert np.allclose(out[0], v, atol=1e-6)
+    v_rev = vel[0].astype(np.float64)
+    for j in (2, 1):
+        d = pos[j].astype(np.float64) - pos[0]
+        nrm = d / np.linalg.norm(d)
+        v_rev = v_rev - 2 * nrm.dot(v_rev) * nrm
+    assert not np.allclose(out[0], v_rev, atol=1e-3)
+    assert abs(np.linalg.norm(out[0]) - np.linalg.norm(vel[0])) < 1e-6
+
+
+def test_collision_random_cloud_vs_float64(orc):
+    rng = np.random.default_rng(8)
+    n = 300
+    pos = rng.uniform(0, 0.2, (n, 3)).astype(np.float32)
+    vel = rng.normal(size=(n, 3)).astype(np.float32)
+    out = orc.collision(pos, vel)
+    ref = vel.astype(np.float64).copy()
+    p64 = pos.astype(np.float64)
+    for i in range(n):
+        v = ref[i].copy()
+        for j in range(n):
+            if j == i:
+                continue
+            d = p64[j] - p64[i]
+            dist = np.linalg.norm(d)
+            if dist <= 0.03:
+                nrm = d / dist
+                v = v - 2 * nrm.dot(v) * nrm
+        ref[i] = v
+    hit = np.any(out != vel, axis=1)
+    assert hit.sum() > 20
+    assert np.allclose(out, ref, atol=2e-5 * np.abs(ref).max())
