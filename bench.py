@@ -96,6 +96,8 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (each box has one MI355X); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy_mib:
+        calibrate_copy(args.calibrate_copy_mib)
     {"bh1m": bench_bh, "ensemble": bench_ensemble, "precision": bench_precision, "serving": bench_serving,
      "bf4096": bench_bf4096}[args.workload](args)
 
@@ -109,8 +111,6 @@ def bench_bh(args):
     info = device_info()
     if info is None:
         raise SystemExit("no HIP device")
-    if args.calibrate_copy_mib:
-        calibrate_copy(args.calibrate_copy_mib)
     rules = L.RULES_STANDARD if args.rules == "standard" else L.RULES_REFERENCE
     n, theta, eps, G, dt = args.n, args.theta, 0.01, 1.0, 1e-3
     log(f"bench: device {info}, N={n}, rules={args.rules}, theta={theta}")
@@ -136,8 +136,17 @@ def bench_bh(args):
 
     # state at the end of the timed region, and one more force pass on it
     sim.force_pipeline()
-    p_end, _, a_end = sim.download()
+    t_dl = time.perf_counter()
+    p_end, v_end, a_end = sim.download()
+    t_dl = time.perf_counter() - t_dl
+    # host round trip through the C ABI (pageable buffers): what a caller that hands over
+    # host arrays every step would pay on top of the resident-state rate
+    t_ul = time.perf_counter()
+    sim.upload(p_end, v_end, mass)
+    t_ul = time.perf_counter() - t_ul
     sim.close()
+    pcie = {"upload_ms": 1e3 * t_ul, "download_ms": 1e3 * t_dl,
+            "steps_per_s_with_host_round_trip": 1.0 / (elapsed / args.steps + t_ul + t_dl)}
 
     from oracle import oracle as orc
 
@@ -203,6 +212,7 @@ def bench_bh(args):
         "ginteractions_executed_per_s": exec_inter * steps_per_s / 1e9,
         "interactions_per_step": {"algorithmic_per_body_traversal": i_alg, "executed_wave_vote": exec_inter},
         "stage_ms_per_step": {k: v / args.steps for k, v in stages.items()},
+        "host_round_trip": pcie,
         "force_error": out_err,
         "roofline": {"bound": "mfma", "note": "FP32 vector ALU (no MFMA); peak = MI355X FP32 157.3 TFLOP/s",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",

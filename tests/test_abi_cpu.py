@@ -188,3 +188,14 @@ def test_reference_ic_modes():
     pos, vel, mass = ic.reference(20000, 4)
     assert abs(pos.std() - 1.5) < 0.03
     assert abs(np.abs(mass).mean() - 0.5) < 0.02
+
+
+
+def test_cpp_binding_without_device(cpp_binding_exe):
+    """The reference-side C++ binding compiles, links, and reports the missing device."""
+    import subprocess
+    from nbodysimulation_amd import device_info
+    if device_info() is not None:
+        pytest.skip("a device is present (the GPU variant runs instead)")
+    out = subprocess.run([cpp_binding_exe], check=True, capture_output=True, text=True).stdout
+    assert out.startswith("stages=2 app=error")  # NB_E_NO_DEVICE, nbody::Error from App

@@ -91,3 +91,10 @@ def test_checkpoint_rejects_mismatch(gpu):
             t.restore(blob)
         with pytest.raises(L.NbError):
             t.restore(b"\0" * 64)
+
+
+def test_cpp_binding_on_gpu(gpu, cpp_binding_exe):
+    """The reference-side C++ binding (stage calls + nbody::App) runs end to end on the device."""
+    import subprocess
+    out = subprocess.run([cpp_binding_exe], check=True, capture_output=True, text=True, timeout=120).stdout
+    assert out.strip() == "stages=0 app=ok n=2048"
