@@ -205,7 +205,7 @@ def bench_bh(args):
         "dtype": "f32",
         "data": "synthetic: seeded Plummer sphere (AHW 1974), generated by the library",
         "config": {"workload": f"barnes-hut Plummer N={n} theta={theta} eps={eps} G={G} dt={dt} Euler seed 1",
-                   "rules": args.rules, "vote_group": 64, "cell_capacity": tinfo["capacity"],
+                   "rules": args.rules, "vote_group": L.vote_group(), "cell_capacity": tinfo["capacity"],
                    "cells_used": tinfo["num_cells"], "max_depth": tinfo["max_depth"],
                    "device": info},
         "ginteractions_per_s": i_alg * steps_per_s / 1e9,

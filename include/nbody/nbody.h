@@ -109,6 +109,11 @@ const char* nb_last_error_detail(void);
 int32_t nb_abi_version(void);
 /* Device description: compute units and name (name may be NULL). */
 nb_status nb_device_info(int32_t* compute_units, char* name, size_t name_len);
+/* Bodies per opening decision of the Barnes-Hut force kernel: the traversal is shared by
+ * groups of this many consecutive tree-ordered bodies, and a cell is accepted for the
+ * group only if it is far enough for every body of it (the reference votes over its
+ * work-group, calculate_forces.cl:121). */
+int32_t nb_force_vote_group(void);
 /* Cell pool size the reference would allocate for n bodies (src/MyApp.cpp:71-83). */
 int32_t nb_reference_cell_capacity(int32_t n, int32_t compute_units);
 

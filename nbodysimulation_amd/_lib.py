@@ -177,6 +177,7 @@ SIGNATURES = {
     "nb_stage_times": (C.c_int, [_P, _FP]),
     "nb_step_async": (C.c_int, [_P, C.c_int32]),
     "nb_sim_energy": (C.c_int, [_P, _DP]),
+    "nb_force_vote_group": (C.c_int32, []),
     "nb_export_vertices": (C.c_int, [_P, _FP, C.c_int32]),
     "nb_sim_step_count": (C.c_int, [_P, _I64P]),
     "nb_checkpoint_size": (C.c_int, [_P, C.POINTER(C.c_size_t)]),
@@ -221,6 +222,11 @@ def load():
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+def vote_group() -> int:
+    """Bodies per opening decision of the library's Barnes-Hut force kernel."""
+    return int(load().nb_force_vote_group())
 
 
 class NbError(RuntimeError):

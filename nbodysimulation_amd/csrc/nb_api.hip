@@ -56,6 +56,8 @@ extern "C" {
 
 int32_t nb_abi_version(void) { return NB_ABI_VERSION; }
 
+int32_t nb_force_vote_group(void) { return nb::kForceVote; }
+
 const char* nb_last_error_detail(void) { return nb::last_error(); }
 
 const char* nb_status_string(nb_status s) {
@@ -437,8 +439,10 @@ static nb_status read_stage_times(nb_sim* sim, int nsteps) {
             NB_HIPC(hipEventElapsedTime(&ms, e[6], e[7]));
             sim->stage_ms[NB_STAGE_UPDATE] += ms;
         }
-        NB_HIPC(hipEventElapsedTime(&ms, e[7], e[8]));
-        sim->stage_ms[NB_STAGE_COLLISION] += ms;
+        if (sim->cfg.collision) {
+            NB_HIPC(hipEventElapsedTime(&ms, e[7], e[8]));
+            sim->stage_ms[NB_STAGE_COLLISION] += ms;
+        }
     }
     return NB_OK;
 }

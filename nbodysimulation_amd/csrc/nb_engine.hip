@@ -13,7 +13,13 @@ const char* last_error() { return g_err.c_str(); }
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 // Insertion phases of the build: every 1024th body, then every 32nd, then the rest.
-static constexpr int kPhaseStride[Engine::kBuildPhases] = {1024, 32, 1};
+#ifndef NB_PHASE0_STRIDE
+#define NB_PHASE0_STRIDE 1024
+#endif
+#ifndef NB_PHASE1_STRIDE
+#define NB_PHASE1_STRIDE 32
+#endif
+static constexpr int kPhaseStride[Engine::kBuildPhases] = {NB_PHASE0_STRIDE, NB_PHASE1_STRIDE, 1};
 static constexpr int kBuildBlock = 1024;  // k_build workgroup size
 static constexpr int kBoundsPerBlock = 2048;
 
@@ -83,7 +89,7 @@ nb_status Engine::reserve(const Capacity& c, int law_, bool kahan_, void* user_s
     tab_cap[T_BOUNDS] = blocks_bound(c.bodies, kBoundsPerBlock, S);
     tab_cap[T_BODY] = blocks_bound(c.bodies, 256, S);
     tab_cap[T_CELLS] = blocks_bound(c.cells, 256, S);
-    tab_cap[T_GROUPS] = blocks_bound(c.bodies, 64 * kForceWaves, S);
+    tab_cap[T_GROUPS] = blocks_bound(c.bodies, kForceVote * kForceWaves, S);
     tab_cap[T_BRUTE] = blocks_bound(c.bodies, 64, S);
     tab_cap[T_UPDATE] = blocks_bound(c.bodies, 256, S);
     for (int ph = 0; ph < kBuildPhases; ++ph)
@@ -183,7 +189,7 @@ nb_status Engine::activate(const std::vector<Slot>& slots) {
             d.bbox_blocks = (int)t[T_BOUNDS].size() - d.bbox_first;
             add_ranges(t[T_BODY], s, d.n, 256);
             add_ranges(t[T_CELLS], s, d.cap, 256);
-            add_ranges(t[T_GROUPS], s, d.n, 64 * kForceWaves);
+            add_ranges(t[T_GROUPS], s, d.n, kForceVote * kForceWaves);
             for (int ph = 0; ph < kBuildPhases; ++ph)
                 add_ranges(t[T_PHASE0 + ph], s, (d.n + kPhaseStride[ph] - 1) / kPhaseStride[ph], kBuildBlock);
         }

@@ -27,7 +27,8 @@ constexpr int kBfWaves = 16;    // brute-force workgroup: 16 waves of 64
 #ifndef NB_FORCE_WAVES
 #define NB_FORCE_WAVES 4
 #endif
-constexpr int kForceWaves = NB_FORCE_WAVES;  // Barnes-Hut force workgroup: waves of 64 (one vote group each)
+constexpr int kForceWaves = NB_FORCE_WAVES;  // Barnes-Hut force workgroup: waves of 64
+constexpr int kForceVote = 64;  // bodies per opening decision: one wave, one lane per body
 constexpr int kEnergyLen = 9;   // kinetic, potential, p[3], L[3], mass
 constexpr int kEmpty = -1;
 constexpr int kLocked = -2;

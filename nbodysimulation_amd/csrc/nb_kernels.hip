@@ -117,6 +117,13 @@ __device__ __forceinline__ int level_digit(int rules, int lvl, float px, float p
     return d;
 }
 
+// Order LDS accesses of one wave (same-wave LDS operations complete in order).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ bool finite3(float4 p) {
     return isfinite(p.x) && isfinite(p.y) && isfinite(p.z);
 }
@@ -652,13 +659,6 @@ __global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __
 // The self pair is not skipped: its displacement is exactly zero, so it adds
 // exactly zero (eps > 0 keeps the PLUMMER denominator finite; LAW 1 maps a
 // zero displacement to a zero unit vector).
-
-// Order LDS accesses of one wave (same-wave LDS operations complete in order).
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 template <int LAW>
 __device__ __forceinline__ void bf_pair(const float4 q, float px, float py, float pz, float e2, float& fx, float& fy,

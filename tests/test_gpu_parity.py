@@ -7,6 +7,7 @@ thresholds are bit-exact; so is every opening decision (per-body interaction
 counts equal).  Accelerations differ only through the hardware reciprocal
 square root and summation order, so they are held to
 |a_gpu - a_oracle| <= ACC_RTOL * sum_k |term_k| per body (ACC_RTOL = 2e-5).
+The oracle emulates the kernel's vote groups (VG consecutive tree-ordered bodies).
 """
 import numpy as np
 import pytest
@@ -18,6 +19,7 @@ from nbodysimulation_amd.engine import NBodySim, make_config
 pytestmark = pytest.mark.gpu
 
 ACC_RTOL = 2e-5
+VG = L.vote_group()  # bodies per opening decision of the force kernel
 
 
 def say(*a):
@@ -80,7 +82,7 @@ def test_tree_and_force_parity(orc, gpu, rules, kind, n):
         assert box.tobytes() == t.box.tobytes()
         compare_tree(s, t, n)
         assert s.export_dq().tobytes() == orc.dq_table(rules, box, theta, eps).tobytes()
-        a_o, absacc, ni = t.force(theta, eps, G, vote=64)
+        a_o, absacc, ni = t.force(theta, eps, G, vote=VG)
         _, _, a_g = s.download()
         assert s.interactions() == int(ni.sum())
         err = np.linalg.norm(a_g - a_o, axis=1)
@@ -94,7 +96,7 @@ def test_kahan_force_parity(orc, gpu):
     s, box = gpu_tree(pos, vel, mass, 1, accum=L.ACCUM_KAHAN)
     try:
         t = orc.Tree(pos, mass, 1, cap=s.tree_info()["capacity"])
-        a_o, absacc, _ = t.force(0.5, 0.01, 1.0, vote=64, kahan=True)
+        a_o, absacc, _ = t.force(0.5, 0.01, 1.0, vote=VG, kahan=True)
         _, _, a_g = s.download()
         assert (np.linalg.norm(a_g - a_o, axis=1) <= ACC_RTOL * absacc).all()
     finally:
@@ -157,7 +159,7 @@ def test_step_matches_oracle_and_is_deterministic(orc, gpu, rules):
     for a, b in zip(outs[1], outs[3]):
         assert a.tobytes() == b.tobytes()
     t = orc.Tree(pos, mass, rules)
-    a_o, absacc, _ = t.force(theta, eps, G, vote=64)
+    a_o, absacc, _ = t.force(theta, eps, G, vote=VG)
     v_o, p_o = orc.euler(a_o, vel, pos, dt)
     p_g, v_g, a_g = outs[0]
     assert (np.linalg.norm(a_g - a_o, axis=1) <= ACC_RTOL * absacc).all()
@@ -175,11 +177,11 @@ def test_leapfrog_kdk(orc, gpu):
         s.step(1)
         p_g, v_g, a_g = s.download()
     t0 = orc.Tree(pos, mass, 1)
-    a0, _, _ = t0.force(theta, eps, G, vote=64)
+    a0, _, _ = t0.force(theta, eps, G, vote=VG)
     vh = orc.kick(a0, vel, 0.5 * dt)
     p1 = orc.drift(vh, pos, dt)
     t1 = orc.Tree(p1, mass, 1)
-    a1, absacc, _ = t1.force(theta, eps, G, vote=64)
+    a1, absacc, _ = t1.force(theta, eps, G, vote=VG)
     v1 = orc.kick(a1, vh, 0.5 * dt)
     assert np.abs(p_g - p1).max() <= 1e-6 * (1 + np.abs(p1).max())
     assert (np.linalg.norm(a_g - a1, axis=1) <= 1e-4 * absacc + 1e-6).all()
@@ -257,10 +259,10 @@ def test_full_size_tree_exact(orc, gpu, rules):
         t = orc.Tree(pos, mass, rules, cap=s.tree_info()["capacity"])
         assert t.status == orc.OK
         compare_tree(s, t, n)
-        groups = np.random.default_rng(0).choice(n // 64, 64, replace=False).astype(np.int32)
-        a_o, absacc, ni = t.force(0.5, 0.01, 1.0, vote=64, groups=groups)
+        groups = np.random.default_rng(0).choice(n // VG, 64, replace=False).astype(np.int32)
+        a_o, absacc, ni = t.force(0.5, 0.01, 1.0, vote=VG, groups=groups)
         _, _, a_g = s.download()
-        members = t.sorted[(groups[:, None] * 64 + np.arange(64)[None, :]).reshape(-1)]
+        members = t.sorted[(groups[:, None] * VG + np.arange(VG)[None, :]).reshape(-1)]
         err = np.linalg.norm(a_g[members] - a_o[members], axis=1)
         assert (err <= ACC_RTOL * absacc[members]).all()
     finally:

@@ -62,9 +62,9 @@ def test_result_alone_equals_in_mix(gpu, method):
         for spec, steps, st in reqs[6:]:
             ids.append(sv.submit(spec, steps, *st))
         status = drain(sv)
+        assert [status[rid] for rid in ids] == [L.NB_OK] * len(ids)
         assert sv.stats()["body_steps"] == sum(s.num_bodies * k for s, k, _ in reqs)
         got = {rid: sv.fetch(rid) for rid in ids}
-    assert all(status[rid] == L.NB_OK for rid in ids)
     for rid, (spec, steps, st) in zip(ids, reqs):
         alone = run_alone(spec, steps, st, method)
         for a, b in zip(alone, got[rid]):
