@@ -47,15 +47,20 @@ static void add_ranges(std::vector<BlkWork>& t, int sys, int count, int per) {
     for (int b = 0; b < count; b += per) t.push_back(BlkWork{sys, b, std::min(count, b + per), 0});
 }
 
-// Brute force: log2 of the waves that split one target group's sources, from N
-// alone (so a system sums in the same order in any batch): enough for 8192 waves
-// at one target per lane (two 16-wave workgroups per CU on 256 CUs), at most the
-// 16 waves of a workgroup.
-static int bf_split_log2(int n) {
-    const long long groups = (n + 63) / 64;
-    int lg = 0;
-    while (lg < 4 && (groups << lg) < 8192) ++lg;
-    return lg;
+// Brute force shape of one system, from N alone (so a system sums in the same
+// order in any batch): S = 2^lg_s waves split a target group's sources (at most
+// the 16 waves of a workgroup), then W = 2^lg_w lanes split each target's
+// sources (at most 4), until one target per lane gives 8192 waves (two 16-wave
+// workgroups per CU on 256 CUs) or the limits are reached.
+static void bf_shape(int n, int& lg_s, int& lg_w) {
+    lg_s = 0;
+    lg_w = 0;
+    auto waves = [&]() {
+        const long long per_wave = 64 >> lg_w;
+        return ((n + per_wave - 1) / per_wave) << lg_s;
+    };
+    while (lg_s < 4 && waves() < 8192) ++lg_s;
+    while (lg_w < 2 && waves() < 8192) ++lg_w;
 }
 
 static int blocks_bound(long long items, int per, int systems) {
@@ -90,7 +95,7 @@ nb_status Engine::reserve(const Capacity& c, int law_, bool kahan_, void* user_s
     tab_cap[T_BODY] = blocks_bound(c.bodies, 256, S);
     tab_cap[T_CELLS] = blocks_bound(c.cells, 256, S);
     tab_cap[T_GROUPS] = blocks_bound(c.bodies, kForceVote * kForceWaves, S);
-    tab_cap[T_BRUTE] = blocks_bound(c.bodies, 64, S);
+    tab_cap[T_BRUTE] = blocks_bound(c.bodies, 16, S);
     tab_cap[T_UPDATE] = blocks_bound(c.bodies, 256, S);
     for (int ph = 0; ph < kBuildPhases; ++ph)
         tab_cap[T_PHASE0 + ph] = blocks_bound((c.bodies + kPhaseStride[ph] - 1) / kPhaseStride[ph], kBuildBlock, S);
@@ -195,8 +200,9 @@ nb_status Engine::activate(const std::vector<Slot>& slots) {
         }
         if (!bh && d.n > 0) {
             any_bf = true;
-            d.bf_split_log2 = bf_split_log2(d.n);
-            bf_waves2 += (long long)((d.n + 127) / 128) << d.bf_split_log2;
+            bf_shape(d.n, d.bf_split_log2, d.bf_lanes_log2);
+            const long long per_wave2 = 128 >> d.bf_lanes_log2;
+            bf_waves2 += ((d.n + per_wave2 - 1) / per_wave2) << d.bf_split_log2;
         }
         add_ranges(t[T_UPDATE], s, d.n, 256);
     }
@@ -205,7 +211,7 @@ nb_status Engine::activate(const std::vector<Slot>& slots) {
     for (const Slot& sl : slots) {
         const SysDesc& d = hdesc[sl.slot];
         if (d.algo != NB_BARNES_HUT && d.n > 0)
-            add_ranges(t[T_BRUTE], sl.slot, d.n, (kBfWaves >> d.bf_split_log2) * 64 * bf_tb);
+            add_ranges(t[T_BRUTE], sl.slot, d.n, (kBfWaves >> d.bf_split_log2) * (64 >> d.bf_lanes_log2) * bf_tb);
     }
     for (int k = 0; k < T_COUNT; ++k) {
         if ((int)t[k].size() > tab_cap[k]) {

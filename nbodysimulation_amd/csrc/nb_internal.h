@@ -24,6 +24,7 @@ namespace nb {
 constexpr int kMaxLevel = 64;          // deepest cell level (root = 0)
 constexpr int kDqLen = kMaxLevel + 1;  // opening thresholds per level
 constexpr int kBfWaves = 16;    // brute-force workgroup: 16 waves of 64
+constexpr int kBfTileStride = 4 * 17;  // a wave's LDS tile: up to 4 parts of 16 sources, rows padded
 #ifndef NB_FORCE_WAVES
 #define NB_FORCE_WAVES 4
 #endif
@@ -54,7 +55,8 @@ struct SysDesc {
     int bbox_first;   // first global block of this system in the bounds table
     int bbox_blocks;  // number of bounds blocks of this system
     int bf_split_log2;  // brute force: log2 of the waves sharing one target group
-    int pad[2];
+    int bf_lanes_log2;  // brute force: log2 of the lanes sharing one target
+    int pad;
 };
 
 struct SysState {

@@ -649,13 +649,15 @@ __global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __
 // LAW 0: d*m*rsqrt(|d|^2+eps^2)^3.  LAW 1 (brute_force_update.cl:54-58,114-120):
 // n = normalize(d), n*m/(|n|^2+ueps^2).
 //
-// Workgroup = 16 waves.  A system's targets are cut into groups of 64*TB (each
-// lane holds TB targets 64 apart: register blocking).  S = d.bf_split waves
-// share one group; wave `part` sums the sources [n*part/S, n*(part+1)/S) in
-// order, staged 64 at a time through its own LDS tile and read back as
-// broadcasts; the S partial sums are then added in wave order through LDS.
-// S depends on N only, and TB does not change any target's arithmetic, so a
-// system's result does not depend on the batch it runs in.
+// Workgroup = 16 waves.  A wave holds 64*TB/W targets: W lanes per target
+// (W = 2^bf_lanes_log2) and TB targets per lane (register blocking).  S =
+// 2^bf_split_log2 waves share one target group; wave `part` sums the sources
+// [n*part/S, n*(part+1)/S), staged 64 at a time through its own LDS tile, lane
+// part p taking tile entries [64p/W, 64(p+1)/W) in order and reading them as
+// broadcasts.  The W parts are added in part order across lanes, then the S
+// waves' sums in wave order through LDS.  W and S depend on N only (small N
+// splits the sources finer so that every CU gets work), and TB does not change
+// any target's arithmetic, so a system's result does not depend on its batch.
 // The self pair is not skipped: its displacement is exactly zero, so it adds
 // exactly zero (eps > 0 keeps the PLUMMER denominator finite; LAW 1 maps a
 // zero displacement to a zero unit vector).
@@ -687,18 +689,17 @@ __device__ __forceinline__ void kahan_add(float& a, float& c, float v) {
     a = t;
 }
 
-template <int LAW, bool KAHAN, int TB>
-__global__ __launch_bounds__(1024) void k_brute(const SysDesc* __restrict__ sd, const SysState* __restrict__ st,
-                                                const BlkWork* __restrict__ work,
-                                                const float4* __restrict__ node4, float4* __restrict__ acc4) {
+// One workgroup's share of a system: W lanes per target (parts of the wave's
+// sources), TB targets per lane, S = 2^lg waves per target group.
+template <int LAW, bool KAHAN, int TB, int W>
+__device__ __forceinline__ void brute_body(const BlkWork& w, const SysDesc& d, const float4* __restrict__ node4,
+                                           float4* __restrict__ acc4, float4 (*tile)[kBfTileStride],
+                                           float (*red)[2 * 6][64]) {
     constexpr int NC = KAHAN ? 6 : 3;
-    __shared__ float4 tile[kBfWaves][64];
-    __shared__ float red[kBfWaves][NC * TB][64];
-    const BlkWork w = work[blockIdx.x];
-    const SysDesc d = sd[w.sys];
-    if (st[w.sys].status) return;  // whole workgroup
+    constexpr int L = 64 / W;  // lanes (targets) per part, sources per part and tile
     // the wave index is uniform over the wave: keep it (and the loop bounds) scalar
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int pw = lane / L, tl = lane % L;
     const int lg = d.bf_split_log2;
     const int S = 1 << lg;
     const int grp = wave >> lg, part = wave & (S - 1);
@@ -706,24 +707,30 @@ __global__ __launch_bounds__(1024) void k_brute(const SysDesc* __restrict__ sd, 
     const float4* __restrict__ nodes = node4 + d.node_off;
     const float e = LAW == 0 ? d.eps : d.ueps;
     const float e2 = e * e;
+    const int gbase = w.begin + grp * L * TB;
     float px[TB], py[TB], pz[TB], acc[NC][TB];
 #pragma unroll
     for (int t = 0; t < TB; ++t) {
-        const int i = min(w.begin + grp * 64 * TB + t * 64 + lane, w.end - 1);
+        const int i = min(gbase + t * L + tl, w.end - 1);
         const float4 p = nodes[i];
         px[t] = p.x; py[t] = p.y; pz[t] = p.z;
 #pragma unroll
         for (int c = 0; c < NC; ++c) acc[c][t] = 0.0f;
     }
+    // part pw of a 64-source tile sits at row pw (rows padded by one entry: no bank conflicts)
+    float4* tw = tile[wave];
+    const int store_at = (lane / L) * (L + 1) + lane % L;
+    const float4* mine = tw + pw * (L + 1);
     const int j0 = (int)((long long)n * part >> lg), j1 = (int)((long long)n * (part + 1) >> lg);
     for (int jt = j0; jt < j1; jt += 64) {
         const int cnt = min(64, j1 - jt);
         wave_sync();
-        if (lane < cnt) tile[wave][lane] = nodes[jt + lane];
+        if (lane < cnt) tw[store_at] = nodes[jt + lane];
         wave_sync();
+        const int my = min(max(cnt - pw * L, 0), L);
 #pragma unroll 4
-        for (int k = 0; k < cnt; ++k) {
-            const float4 q = tile[wave][k];
+        for (int k = 0; k < my; ++k) {
+            const float4 q = mine[k];
 #pragma unroll
             for (int t = 0; t < TB; ++t) {
                 float fx, fy, fz, sc;
@@ -736,6 +743,31 @@ __global__ __launch_bounds__(1024) void k_brute(const SysDesc* __restrict__ sd, 
                     acc[0][t] = fmaf(fx, sc, acc[0][t]);
                     acc[1][t] = fmaf(fy, sc, acc[1][t]);
                     acc[2][t] = fmaf(fz, sc, acc[2][t]);
+                }
+            }
+        }
+    }
+    if (W > 1) {  // parts of the wave, added in part order into the part-0 lanes
+#pragma unroll
+        for (int t = 0; t < TB; ++t) {
+            float v[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[c] = KAHAN ? acc[c][t] - acc[c + 3][t] : acc[c][t];
+            float o[W][3];
+#pragma unroll
+            for (int q = 1; q < W; ++q)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) o[q][c] = __shfl(v[c], tl + q * L, 64);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (KAHAN) {
+                    acc[c][t] = v[c];
+                    acc[c + 3][t] = 0.0f;
+#pragma unroll
+                    for (int q = 1; q < W; ++q) kahan_add(acc[c][t], acc[c + 3][t], o[q][c]);
+                } else {
+#pragma unroll
+                    for (int q = 1; q < W; ++q) acc[c][t] += o[q][c];
                 }
             }
         }
@@ -764,14 +796,31 @@ __global__ __launch_bounds__(1024) void k_brute(const SysDesc* __restrict__ sd, 
             }
         }
     }
+    if (pw != 0) return;
 #pragma unroll
     for (int t = 0; t < TB; ++t) {
-        const int i = w.begin + grp * 64 * TB + t * 64 + lane;
+        const int i = gbase + t * L + tl;
         if (i < w.end) {
             float ax = acc[0][t], ay = acc[1][t], az = acc[2][t];
             if (KAHAN) { ax -= acc[3][t]; ay -= acc[4][t]; az -= acc[5][t]; }
             acc4[d.body_off + i] = make_float4(ax * d.G, ay * d.G, az * d.G, (float)(n - 1));
         }
+    }
+}
+
+template <int LAW, bool KAHAN, int TB>
+__global__ __launch_bounds__(1024) void k_brute(const SysDesc* __restrict__ sd, const SysState* __restrict__ st,
+                                                const BlkWork* __restrict__ work,
+                                                const float4* __restrict__ node4, float4* __restrict__ acc4) {
+    __shared__ float4 tile[kBfWaves][kBfTileStride];
+    __shared__ float red[kBfWaves][2 * 6][64];
+    const BlkWork w = work[blockIdx.x];
+    const SysDesc d = sd[w.sys];
+    if (st[w.sys].status) return;  // whole workgroup
+    switch (d.bf_lanes_log2) {     // uniform over the workgroup
+    case 0: brute_body<LAW, KAHAN, TB, 1>(w, d, node4, acc4, tile, red); break;
+    case 1: brute_body<LAW, KAHAN, TB, 2>(w, d, node4, acc4, tile, red); break;
+    default: brute_body<LAW, KAHAN, TB, 4>(w, d, node4, acc4, tile, red); break;
     }
 }
 
