@@ -72,7 +72,10 @@ constexpr size_t kBytesPerBody = 16 + 16 + 4;            // vel4, acc4, sorted
 constexpr size_t kBytesPerCell = 32 + 4 * 3 + 32 + 128;  // child, parent/pend/count, slotoff, cdata
 
 struct Engine {
-    static constexpr int kBuildPhases = 3;
+#ifndef NB_BUILD_PHASES
+#define NB_BUILD_PHASES 3
+#endif
+    static constexpr int kBuildPhases = NB_BUILD_PHASES;  // insertion phases of the build (at most 4)
 
     int device = 0;
     int cus = 0;
@@ -99,7 +102,8 @@ struct Engine {
     int nactive = 0;
 
     // block tables (device) with their capacities and current lengths
-    enum Table { T_BOUNDS, T_BODY, T_CELLS, T_GROUPS, T_BRUTE, T_UPDATE, T_PHASE0, T_PHASE1, T_PHASE2, T_COUNT };
+    enum Table { T_BOUNDS, T_BODY, T_CELLS, T_GROUPS, T_BRUTE, T_UPDATE, T_PHASE0, T_PHASE1, T_PHASE2, T_PHASE3,
+                 T_COUNT };
     BlkWork* tab[T_COUNT] = {};
     int tab_cap[T_COUNT] = {};
     int tab_len[T_COUNT] = {};

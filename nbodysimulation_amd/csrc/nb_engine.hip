@@ -13,13 +13,10 @@ const char* last_error() { return g_err.c_str(); }
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 // Insertion phases of the build: every 1024th body, then every 32nd, then the rest.
-#ifndef NB_PHASE0_STRIDE
-#define NB_PHASE0_STRIDE 1024
+#ifndef NB_PHASE_STRIDES
+#define NB_PHASE_STRIDES 1024, 32, 1
 #endif
-#ifndef NB_PHASE1_STRIDE
-#define NB_PHASE1_STRIDE 32
-#endif
-static constexpr int kPhaseStride[Engine::kBuildPhases] = {NB_PHASE0_STRIDE, NB_PHASE1_STRIDE, 1};
+static constexpr int kPhaseStride[Engine::kBuildPhases] = {NB_PHASE_STRIDES};
 static constexpr int kBuildBlock = 1024;  // k_build workgroup size
 static constexpr int kBoundsPerBlock = 2048;
 
