@@ -343,13 +343,14 @@ def bench_precision(args):
     direct sum on 1024 seeded targets, then 200 steps each with FP64 energy / momentum /
     angular momentum computed on the GPU before and after."""
     from nbodysimulation_amd import _lib as L
-    from nbodysimulation_amd import device_info, ic
+    from nbodysimulation_amd import autotune, device_info, ic
     from nbodysimulation_amd.engine import NBodySim, make_config
     from oracle import oracle as orc
 
     info = device_info()
     if info is None:
         raise SystemExit("no HIP device")
+    shape = autotune.load_or_tune(0.5, 0.01, log=log)["brute_shape"]
     n, eps, G, dt, steps = 131072, 0.005, 1.0, 1e-3, args.steps
     pos, vel, mass = ic.collision(n, 3)
     tg = np.random.default_rng(12345).choice(n, 1024, replace=False).astype(np.int32)
@@ -408,7 +409,8 @@ def bench_precision(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic: two seeded Plummer spheres (library generator)",
         "config": {"workload": f"brute force, two colliding Plummer spheres N={n} eps={eps} G={G} dt={dt} Euler "
-                               f"seed 3", "device": info},
+                               f"seed 3", "device": info,
+                   "brute_shape": {k: shape[k] for k in ("bf_targets_per_lane", "bf_min_waves")}},
         "ginteractions_per_s": float(n) * n * out["plain"]["steps_per_s"] / 1e9,
         "plain": out["plain"], "kahan": out["kahan"],
         "roofline": {"bound": "mfma", "note": "FP32 vector ALU (no MFMA); peak = MI355X FP32 157.3 TFLOP/s",
@@ -528,13 +530,14 @@ def bench_bf4096(args):
     dt = 1e-3, Euler, seed 0, the reference update kernel's law (unit vectors, e = 0.01); the
     Plummer-softened law (eps = 0.01) is timed beside it.  Steps replay a captured HIP graph."""
     from nbodysimulation_amd import _lib as L
-    from nbodysimulation_amd import device_info, ic
+    from nbodysimulation_amd import autotune, device_info, ic
     from nbodysimulation_amd.engine import NBodySim, make_config
     from oracle import oracle as orc
 
     info = device_info()
     if info is None:
         raise SystemExit("no HIP device")
+    shape = autotune.load_or_tune(0.5, 0.01, log=log)["brute_shape"]
     n, dt, steps = 4096, 1e-3, args.steps
     pos, vel, mass = ic.normal_cloud(n, 0)
     res = {}
@@ -575,7 +578,7 @@ def bench_bf4096(args):
         "ms_per_step": 1e3 / r["steps_per_s"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic: seeded Normal(0,1) cloud, m ~ U[0.5,1.5]/N (library generator)",
         "config": {"workload": "plain brute force N=4096, Normal(0,1), G=1, dt=1e-3, 100 steps, seed 0",
-                   "device": info},
+                   "device": info, "brute_shape": {k: shape[k] for k in ("bf_targets_per_lane", "bf_min_waves")}},
         "laws": res,
         "roofline": {"bound": "mfma", "note": "FP32 vector ALU (no MFMA); peak = MI355X FP32 157.3 TFLOP/s; "
                                               "at N=4096 one launch is a few microseconds",

@@ -114,6 +114,19 @@ nb_status nb_device_info(int32_t* compute_units, char* name, size_t name_len);
  * group only if it is far enough for every body of it (the reference votes over its
  * work-group, calculate_forces.cl:121). */
 int32_t nb_force_vote_group(void);
+/* Launch-shape tuning (no counterpart in the reference; set by the autotuner).  Process-wide;
+ * applies to simulations, ensembles and servers whose batch is (re)described afterwards.  A
+ * system's arithmetic depends on bf_min_waves (how finely its sources are split), so results
+ * are bit-identical across batches only under the same tuning. */
+typedef struct nb_tuning {
+    int32_t bf_targets_per_lane; /* brute-force register blocking: 0 automatic, 1 or 2 */
+    int32_t bf_min_waves;        /* brute force: split a system's sources until it yields this many
+                                    waves (at most 16 waves x 4 lanes per target); default 8192 */
+    int32_t reserved[6];
+} nb_tuning;
+void nb_tuning_default(nb_tuning* t);
+nb_status nb_set_tuning(const nb_tuning* t);
+nb_status nb_get_tuning(nb_tuning* t);
 /* Cell pool size the reference would allocate for n bodies (src/MyApp.cpp:71-83). */
 int32_t nb_reference_cell_capacity(int32_t n, int32_t compute_units);
 

@@ -98,6 +98,14 @@ CHECKPOINT_MAGIC = 0x4B43424E
 CHECKPOINT_VERSION = 1
 
 
+class NbTuning(C.Structure):
+    _fields_ = [
+        ("bf_targets_per_lane", C.c_int32),
+        ("bf_min_waves", C.c_int32),
+        ("reserved", C.c_int32 * 6),
+    ]
+
+
 class NbServerStats(C.Structure):
     _fields_ = [
         ("active", C.c_int32),
@@ -178,6 +186,9 @@ SIGNATURES = {
     "nb_step_async": (C.c_int, [_P, C.c_int32]),
     "nb_sim_energy": (C.c_int, [_P, _DP]),
     "nb_force_vote_group": (C.c_int32, []),
+    "nb_tuning_default": (None, [C.POINTER(NbTuning)]),
+    "nb_set_tuning": (C.c_int, [C.POINTER(NbTuning)]),
+    "nb_get_tuning": (C.c_int, [C.POINTER(NbTuning)]),
     "nb_export_vertices": (C.c_int, [_P, _FP, C.c_int32]),
     "nb_sim_step_count": (C.c_int, [_P, _I64P]),
     "nb_checkpoint_size": (C.c_int, [_P, C.POINTER(C.c_size_t)]),
@@ -222,6 +233,23 @@ def load():
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+def get_tuning() -> dict:
+    t = NbTuning()
+    check(load().nb_get_tuning(C.byref(t)), "nb_get_tuning")
+    return {"bf_targets_per_lane": t.bf_targets_per_lane, "bf_min_waves": t.bf_min_waves}
+
+
+def set_tuning(bf_targets_per_lane: int | None = None, bf_min_waves: int | None = None) -> None:
+    """Process-wide launch-shape tuning; unspecified fields keep the library defaults."""
+    t = NbTuning()
+    load().nb_tuning_default(C.byref(t))
+    if bf_targets_per_lane is not None:
+        t.bf_targets_per_lane = int(bf_targets_per_lane)
+    if bf_min_waves is not None:
+        t.bf_min_waves = int(bf_min_waves)
+    check(load().nb_set_tuning(C.byref(t)), "nb_set_tuning")
 
 
 def vote_group() -> int:

@@ -45,6 +45,9 @@ void launch_update(hipStream_t s, int nblk, const SysDesc* sd, const SysState* s
 void set_error(const std::string& msg);
 const char* last_error();
 
+// Process-wide launch-shape tuning (nb_set_tuning).
+const nb_tuning& tuning();
+
 // One system, as requested by the caller.
 struct SysSpec {
     int n = 0;

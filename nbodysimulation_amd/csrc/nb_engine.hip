@@ -44,11 +44,46 @@ static void add_ranges(std::vector<BlkWork>& t, int sys, int count, int per) {
     for (int b = 0; b < count; b += per) t.push_back(BlkWork{sys, b, std::min(count, b + per), 0});
 }
 
-// Brute force shape of one system, from N alone (so a system sums in the same
-// order in any batch): S = 2^lg_s waves split a target group's sources (at most
-// the 16 waves of a workgroup), then W = 2^lg_w lanes split each target's
-// sources (at most 4), until one target per lane gives 8192 waves (two 16-wave
-// workgroups per CU on 256 CUs) or the limits are reached.
+static nb_tuning g_tuning = {0, 8192, {0, 0, 0, 0, 0, 0}};
+const nb_tuning& tuning() { return g_tuning; }
+
+}  // namespace nb
+
+extern "C" {
+
+void nb_tuning_default(nb_tuning* t) {
+    if (!t) return;
+    *t = nb_tuning{0, 8192, {0, 0, 0, 0, 0, 0}};
+}
+
+nb_status nb_set_tuning(const nb_tuning* t) {
+    if (!t || t->bf_targets_per_lane < 0 || t->bf_targets_per_lane > 2 || t->bf_min_waves < 1 ||
+        t->bf_min_waves > (1 << 24)) {
+        nb::set_error("bad tuning");
+        return NB_E_INVALID_ARG;
+    }
+    nb::g_tuning = *t;
+    return NB_OK;
+}
+
+nb_status nb_get_tuning(nb_tuning* t) {
+    if (!t) {
+        nb::set_error("NULL tuning");
+        return NB_E_INVALID_ARG;
+    }
+    *t = nb::g_tuning;
+    return NB_OK;
+}
+
+}  // extern "C"
+
+namespace nb {
+
+// Brute force shape of one system, from N and the tuning alone (so a system sums
+// in the same order in any batch): S = 2^lg_s waves split a target group's sources
+// (at most the 16 waves of a workgroup), then W = 2^lg_w lanes split each target's
+// sources (at most 4), until one target per lane gives bf_min_waves waves (default
+// 8192: two 16-wave workgroups per CU on 256 CUs) or the limits are reached.
 static void bf_shape(int n, int& lg_s, int& lg_w) {
     lg_s = 0;
     lg_w = 0;
@@ -56,8 +91,9 @@ static void bf_shape(int n, int& lg_s, int& lg_w) {
         const long long per_wave = 64 >> lg_w;
         return ((n + per_wave - 1) / per_wave) << lg_s;
     };
-    while (lg_s < 4 && waves() < 8192) ++lg_s;
-    while (lg_w < 2 && waves() < 8192) ++lg_w;
+    const long long target = g_tuning.bf_min_waves;
+    while (lg_s < 4 && waves() < target) ++lg_s;
+    while (lg_w < 2 && waves() < target) ++lg_w;
 }
 
 static int blocks_bound(long long items, int per, int systems) {
@@ -204,7 +240,8 @@ nb_status Engine::activate(const std::vector<Slot>& slots) {
         add_ranges(t[T_UPDATE], s, d.n, 256);
     }
     // two targets per lane only when the batch still fills every CU twice over
-    bf_tb = bf_waves2 >= 8192 ? 2 : 1;
+    bf_tb = g_tuning.bf_targets_per_lane ? g_tuning.bf_targets_per_lane
+                                         : (bf_waves2 >= g_tuning.bf_min_waves ? 2 : 1);
     for (const Slot& sl : slots) {
         const SysDesc& d = hdesc[sl.slot];
         if (d.algo != NB_BARNES_HUT && d.n > 0)

@@ -94,3 +94,34 @@ def test_energy_matches_oracle(orc, gpu, algo, law, n):
     assert np.abs(e["momentum"] - o["momentum"]).max() <= 1e-12 * scale
     lscale = float((mass * np.linalg.norm(pos, axis=1) * np.linalg.norm(vel, axis=1)).sum()) + 1e-300
     assert np.abs(e["angular"] - o["angular"]).max() <= 1e-12 * lscale
+
+
+@pytest.mark.parametrize("tpl,waves", [(1, 4096), (2, 16384)])
+def test_tuned_brute_shape_keeps_parity(orc, gpu, tpl, waves):
+    """Other launch shapes (as the autotuner may pick) stay within tolerance of the oracle and
+    bit-identical between a system alone and in a batch."""
+    L.set_tuning(bf_targets_per_lane=tpl, bf_min_waves=waves)
+    try:
+        sizes = [4096, 777, 20000]
+        ics = [ic.normal_cloud(n, 90 + k) for k, n in enumerate(sizes)]
+        specs = [system_spec(n, L.BRUTE_FORCE_GLOBAL, eps=0.01) for n in sizes]
+        with NBodyEnsemble(specs, law=L.LAW_PLUMMER) as ens:
+            for k, (p, v, m) in enumerate(ics):
+                ens.upload(k, p, v, m)
+            ens.step(1)
+            mixed = [ens.download(k) for k in range(len(sizes))]
+        for k, (n, (p, v, m)) in enumerate(zip(sizes, ics)):
+            with NBodySim(make_config(n, L.BRUTE_FORCE_GLOBAL, law=L.LAW_PLUMMER, eps=0.01)) as s:
+                s.upload(p, v, m)
+                s.brute_force()
+                _, _, a = s.download()
+                s.upload(p, v, m)
+                s.step(1)
+                alone = s.download()
+            a_o, absacc = orc.force_bf(p, m, L.LAW_PLUMMER, 0.01, 1.0)
+            assert (np.linalg.norm(a - a_o, axis=1) <= ACC_RTOL * absacc).all()
+            for x, y in zip(alone, mixed[k]):
+                assert x.tobytes() == y.tobytes(), f"system {k} differs"
+    finally:
+        L.set_tuning()
+    assert L.get_tuning() == {"bf_targets_per_lane": 0, "bf_min_waves": 8192}
