@@ -11,6 +11,8 @@
 //   k_brute       update / update_local all-pairs            kernels/brute_force_update.cl:3-228
 //   k_update      integrator branch                          kernels/calculate_forces.cl:150-182
 //   k_collision   update_collision_local                     kernels/collision_update.cl:3-83
+//   k_vertices    copy_vertices                              kernels/copy_vertices.cl:3-17
+//   k_energy_f64, k_energy_final   FP64 conservation diagnostics (no counterpart)
 //
 // Memory-model notes (gfx950 has one L2 per XCD):
 //   * Data that one workgroup writes and another reads inside the same kernel is

@@ -282,7 +282,10 @@ nb_status nb_server_tick(nb_server* sv, int32_t* completed) {
         }
         if (!try_place(sv, r)) break;
         nb_status st = upload_request(sv, r);
-        if (st) return st;
+        if (st) {  // the request stays queued; its ranges go back to the pools
+            unplace(sv, r);
+            return st;
+        }
         fresh.push_back(r.slot);
         sv->active.push_back(std::move(sv->queue.front()));
         sv->queue.pop_front();
