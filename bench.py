@@ -402,6 +402,11 @@ def bench_precision(args):
                          f"{t_cpu:.2f} s per step"}
     flops = FLOPS_PER_INTERACTION * float(n) * n
     achieved = flops / (out["plain"]["brute_kernel_ms"] * 1e-3) / 1e12
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_brute_bf_precision_131072.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("plain", {}).get("bytes_per_launch")
     line = {
         "metric": "brute-force timesteps/s at N=131072 (plain FP32; Kahan and FP64 diagnostics beside)",
         "value": out["plain"]["steps_per_s"], "unit": "timesteps/s", "n_gpus": 1, "steps": steps,
@@ -415,7 +420,7 @@ def bench_precision(args):
         "plain": out["plain"], "kahan": out["kahan"],
         "roofline": {"bound": "mfma", "note": "FP32 vector ALU (no MFMA); peak = MI355X FP32 157.3 TFLOP/s",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel": "k_brute",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic, "kernel": "k_brute",
                      "kernel_ms": out["plain"]["brute_kernel_ms"], "flops_per_launch": flops},
         "cpu_baseline": cpu,
     }

@@ -199,3 +199,51 @@ def test_cpp_binding_without_device(cpp_binding_exe):
         pytest.skip("a device is present (the GPU variant runs instead)")
     out = subprocess.run([cpp_binding_exe], check=True, capture_output=True, text=True).stdout
     assert out.startswith("stages=2 app=error")  # NB_E_NO_DEVICE, nbody::Error from App
+
+
+def test_tuning_roundtrip_and_validation(nbl):
+    from nbodysimulation_amd import _lib as L
+    assert L.get_tuning() == {"bf_targets_per_lane": 0, "bf_min_waves": 8192}
+    try:
+        L.set_tuning(bf_targets_per_lane=2, bf_min_waves=4096)
+        assert L.get_tuning() == {"bf_targets_per_lane": 2, "bf_min_waves": 4096}
+        for bad in (dict(bf_targets_per_lane=3), dict(bf_min_waves=0)):
+            with pytest.raises(L.NbError):
+                L.set_tuning(**bad)
+    finally:
+        L.set_tuning()
+    assert L.get_tuning() == {"bf_targets_per_lane": 0, "bf_min_waves": 8192}
+
+
+def test_server_and_ensemble_without_device_report_no_device(nbl):
+    from nbodysimulation_amd import _lib as L
+    from nbodysimulation_amd import device_info
+    from nbodysimulation_amd.engine import system_spec
+    if device_info() is not None:
+        pytest.skip("a device is present")
+    h = C.c_void_p()
+    assert nbl.nb_server_create(1 << 30, 4, L.EULER, L.LAW_PLUMMER, L.ACCUM_PLAIN, None, C.byref(h)) == \
+        L.NB_E_NO_DEVICE
+    specs = (L.NbSystemSpec * 2)(system_spec(100), system_spec(200))
+    assert nbl.nb_ens_create(specs, 2, L.EULER, L.LAW_PLUMMER, L.ACCUM_PLAIN, 1, None, C.byref(h)) == \
+        L.NB_E_NO_DEVICE
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_autotune_host_helpers():
+    from nbodysimulation_amd import _lib as L
+    from nbodysimulation_amd import autotune
+    a = autotune.log_uniform_sizes(1000, 4096, 2097152, seed=3)
+    b = autotune.log_uniform_sizes(1000, 4096, 2097152, seed=3)
+    assert np.array_equal(a, b) and a.min() >= 4096 and a.max() <= 2097152
+    # log-uniform: the median sits near the geometric mean of the bounds
+    assert abs(np.log(np.median(a)) - 0.5 * (np.log(4096) + np.log(2097152))) < 0.3
+    assert autotune.choose_algorithm(8191, 8192) == L.BRUTE_FORCE_GLOBAL
+    assert autotune.choose_algorithm(8192, 8192) == L.BARNES_HUT

@@ -267,3 +267,18 @@ def test_full_size_tree_exact(orc, gpu, rules):
         assert (err <= ACC_RTOL * absacc[members]).all()
     finally:
         s.close()
+
+
+def test_largest_served_system_tree_exact(orc, gpu):
+    """N = 2,097,152 (the largest system the serving workload admits), STANDARD rules, with
+    the server's pool sizing (2N + 1024 cells): bit-exact tree and sort against the oracle."""
+    n = 1 << 21
+    say(f"case 2M n={n}")
+    pos, vel, mass = ic.plummer(n, 5)
+    s, box = gpu_tree(pos, vel, mass, 1, theta=0.5, eps=0.01, G=1.0, cap=2 * n + 1024)
+    try:
+        t = orc.Tree(pos, mass, 1, cap=2 * n + 1024)
+        assert t.status == orc.OK
+        compare_tree(s, t, n)
+    finally:
+        s.close()
