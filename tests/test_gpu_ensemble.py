@@ -59,3 +59,48 @@ def test_ensemble_failure_is_per_system(gpu):
         assert st[0] == L.NB_OK and st[1] == L.NB_E_NONFINITE
         p0, _, _ = ens.download(0)
         assert not np.array_equal(p0, good[0])  # the healthy system advanced
+
+
+def test_ensemble_with_empty_and_single_body_systems(gpu):
+    """Systems of 0 and 1 bodies (both algorithms) sit in a batch without disturbing the others."""
+    specs = [system_spec(0), system_spec(1), system_spec(0, L.BRUTE_FORCE_GLOBAL),
+             system_spec(1, L.BRUTE_FORCE_GLOBAL), system_spec(2000)]
+    big = ic.plummer(2000, 8)
+    with NBodyEnsemble(specs, law=L.LAW_PLUMMER) as ens:
+        ens.upload(0, np.zeros((0, 3), np.float32), None, np.zeros(0, np.float32))
+        ens.upload(1, np.array([[0.5, 0.25, -1.0]], np.float32), np.array([[1.0, 0.0, 0.0]], np.float32),
+                   np.array([1.0], np.float32))
+        ens.upload(2, np.zeros((0, 3), np.float32), None, np.zeros(0, np.float32))
+        ens.upload(3, np.array([[0.0, 0.0, 0.0]], np.float32), np.array([[0.0, 2.0, 0.0]], np.float32),
+                   np.array([1.0], np.float32))
+        ens.upload(4, *big)
+        ens.step(3)
+        assert (ens.status() == L.NB_OK).all()
+        p1, v1, a1 = ens.download(1)
+        p3, v3, a3 = ens.download(3)
+        mixed = ens.download(4)
+    # a lone body feels no force and drifts: x += 3 dt v
+    assert not a1.any() and not a3.any()
+    assert np.allclose(p1[0], [0.5 + 3e-3, 0.25, -1.0], atol=1e-6)
+    assert np.allclose(p3[0], [0.0, 6e-3, 0.0], atol=1e-6)
+    with NBodySim(make_config(2000, L.BARNES_HUT, rules=L.RULES_STANDARD, law=L.LAW_PLUMMER,
+                              cell_capacity=2 * 2000 + 1024)) as s:
+        s.upload(*big)
+        s.step(3)
+        alone = s.download()
+    for a, b in zip(alone, mixed):
+        assert a.tobytes() == b.tobytes()
+
+
+def test_step_async_then_status(gpu):
+    pos, vel, mass = ic.plummer(5000, 12)
+    with NBodySim(make_config(5000, L.BARNES_HUT, rules=L.RULES_STANDARD)) as s:
+        s.upload(pos, vel, mass)
+        s.step_async(4)
+        assert s.status() == L.NB_OK
+        assert s.step_count() == 4
+        with NBodySim(make_config(5000, L.BARNES_HUT, rules=L.RULES_STANDARD)) as t:
+            t.upload(pos, vel, mass)
+            t.step(4)
+            for a, b in zip(s.download(), t.download()):
+                assert a.tobytes() == b.tobytes()

@@ -103,3 +103,19 @@ def test_failure_retires_only_that_request(gpu):
     alone = run_alone(system_spec(3000), 4, good, L.EULER)
     for x, y in zip(alone, res):
         assert x.tobytes() == y.tobytes()
+
+
+def test_empty_system_request(gpu):
+    """A request with no bodies is admitted, stepped and retired like any other."""
+    p, v, m = ic.plummer(500, 4)
+    with NBodyServer(8 * MiB, 3) as sv:
+        e = sv.submit(system_spec(0), 5, np.zeros((0, 3), np.float32), None, np.zeros(0, np.float32))
+        a = sv.submit(system_spec(500), 5, p, v, m)
+        status = drain(sv)
+        assert status == {e: L.NB_OK, a: L.NB_OK}
+        pe, ve, ae = sv.fetch(e)
+        assert pe.shape == (0, 3)
+        res = sv.fetch(a)
+    alone = run_alone(system_spec(500), 5, (p, v, m), L.EULER)
+    for x, y in zip(alone, res):
+        assert x.tobytes() == y.tobytes()
