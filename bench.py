@@ -75,6 +75,7 @@ def parse_args():
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--no-shares", action="store_true", help="bh1m: skip the per-GPU target-share timing")
     ap.add_argument("--requests", type=int, default=2000, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=6.0, help="serving: Poisson arrival rate, requests/s")
     ap.add_argument("--budget-gib", type=float, default=96.0, help="serving: HBM budget of the server arena")
@@ -148,6 +149,8 @@ def bench_bh(args):
     pcie = {"upload_ms": 1e3 * t_ul, "download_ms": 1e3 * t_dl,
             "steps_per_s_with_host_round_trip": 1.0 / (elapsed / args.steps + t_ul + t_dl)}
 
+    shares = None if args.no_shares else target_shares(p_end, v_end, mass, rules, theta, eps, G, dt)
+
     from oracle import oracle as orc
 
     out_err = None
@@ -213,6 +216,7 @@ def bench_bh(args):
         "interactions_per_step": {"algorithmic_per_body_traversal": i_alg, "executed_wave_vote": exec_inter},
         "stage_ms_per_step": {k: v / args.steps for k, v in stages.items()},
         "host_round_trip": pcie,
+        "target_shares": shares,
         "force_error": out_err,
         "roofline": {"bound": "mfma", "note": "FP32 vector ALU (no MFMA); peak = MI355X FP32 157.3 TFLOP/s",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
@@ -222,6 +226,50 @@ def bench_bh(args):
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def target_shares(pos, vel, mass, rules, theta, eps, G, dt, gpus=(2, 4, 8), reps=5):
+    """One GPU's share of the force pass when the targets are split over G GPUs that each hold
+    the whole tree: the force stage over [r*N/G, (r+1)*N/G) of the tree order, timed with HIP
+    events on the simulation's stream (mean of `reps` launches after one warm-up), for every rank
+    r.  The all-gather of positions that such a split needs is not part of it."""
+    import ctypes as C
+
+    from nbodysimulation_amd import _lib as L
+    from nbodysimulation_amd.engine import NBodySim, make_config
+
+    n = mass.size
+    stream, hip = hip_stream()
+    ev0, ev1 = C.c_void_p(), C.c_void_p()
+    hip.hipEventCreate(C.byref(ev0))
+    hip.hipEventCreate(C.byref(ev1))
+
+    def timed(launch):
+        launch()
+        hip.hipEventRecord(ev0, C.c_void_p(stream))
+        for _ in range(reps):
+            launch()
+        hip.hipEventRecord(ev1, C.c_void_p(stream))
+        hip.hipEventSynchronize(ev1)
+        ms = C.c_float()
+        hip.hipEventElapsedTime(C.byref(ms), ev0, ev1)
+        return ms.value / reps
+
+    out = {"reps": reps, "order": "tree order, rank r takes [r*N/G, (r+1)*N/G)"}
+    with NBodySim(make_config(n, L.BARNES_HUT, rules=rules, theta=theta, eps=eps, G=G, dt=dt), stream=stream) as sim:
+        sim.upload(pos, vel, mass)
+        sim.force_pipeline()
+        out["full_force_ms"] = timed(sim.force)
+        for g in gpus:
+            ms = [timed(lambda a=n * r // g, b=n * (r + 1) // g: sim.force_range(a, b - a)) for r in range(g)]
+            out[str(g)] = {"share_force_ms": ms, "max_share_force_ms": max(ms), "sum_ms": sum(ms)}
+            log(f"bench: force share at G={g}: max {max(ms):.3f} ms, sum {sum(ms):.3f} ms "
+                f"(full pass {out['full_force_ms']:.3f} ms)")
+        assert sim.status() == L.NB_OK
+    hip.hipEventDestroy(ev0)
+    hip.hipEventDestroy(ev1)
+    hip.hipStreamDestroy(C.c_void_p(stream))
+    return out
 
 
 def hip_stream():

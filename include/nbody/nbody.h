@@ -162,6 +162,19 @@ nb_status nb_stage_force(nb_sim* sim);
 nb_status nb_stage_update(nb_sim* sim);
 /* update / update_local all-pairs force (kernels/brute_force_update.cl:3-228); writes acc. */
 nb_status nb_stage_brute_force(nb_sim* sim);
+/* One GPU's share of a force pass, for callers that shard the targets over G GPUs that
+ * each hold the whole tree (rank r takes [r*N/G, (r+1)*N/G)).  The counterpart is the
+ * same kernel as above (calculate_force_local, kernels/calculate_forces.cl:44-148;
+ * update_local, kernels/brute_force_update.cl:108-228) run over a sub-range of its
+ * NDRange; the reference itself never launches one.  Accelerations of the targets
+ * [first, first+count) are written, every other body's are left as they were.
+ * nb_stage_force_range counts targets in tree order (nb_export_bodies' `sorted`); the
+ * vote groups stay those of the full pass (64 consecutive bodies from tree position 0),
+ * so every written acceleration equals the full pass's bit for bit, wherever the range
+ * starts or ends.  nb_stage_brute_force_range counts targets in body index order; its
+ * results also equal the full pass's.  A range outside [0, N] is NB_E_INVALID_ARG. */
+nb_status nb_stage_force_range(nb_sim* sim, int64_t first, int64_t count);
+nb_status nb_stage_brute_force_range(nb_sim* sim, int64_t first, int64_t count);
 /* update_collision_local (kernels/collision_update.cl:3-83): bodies are spheres of radius
  * 0.015; each body's velocity is reflected about the separation of every other body within
  * 0.03, in index order.  O(N^2); reads positions, writes velocities. */

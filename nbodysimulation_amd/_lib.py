@@ -176,6 +176,8 @@ SIGNATURES = {
     "nb_stage_update": (C.c_int, [_P]),
     "nb_stage_brute_force": (C.c_int, [_P]),
     "nb_stage_collision": (C.c_int, [_P]),
+    "nb_stage_force_range": (C.c_int, [_P, C.c_int64, C.c_int64]),
+    "nb_stage_brute_force_range": (C.c_int, [_P, C.c_int64, C.c_int64]),
     "nb_step": (C.c_int, [_P, C.c_int32]),
     "nb_tree_info_get": (C.c_int, [_P, C.POINTER(NbTreeInfo)]),
     "nb_export_bodies": (C.c_int, [_P, _IP, _U8P, _IP]),

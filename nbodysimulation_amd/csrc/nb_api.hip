@@ -353,6 +353,33 @@ nb_status nb_stage_brute_force(nb_sim* sim) {
     return NB_OK;
 }
 
+static nb_status check_range(const nb_sim* sim, int64_t first, int64_t count) {
+    if (first < 0 || count < 0 || first > sim->n || count > sim->n - first)
+        return fail(NB_E_INVALID_ARG, "target range [" + std::to_string(first) + ", " +
+                                          std::to_string(first + count) + ") outside [0, " + std::to_string(sim->n) + ")");
+    return NB_OK;
+}
+
+nb_status nb_stage_force_range(nb_sim* sim, int64_t first, int64_t count) {
+    if (!sim) return fail(NB_E_INVALID_ARG, "NULL sim");
+    nb_status st = need(sim->have_sorted, "force needs a sorted tree");
+    if (!st) st = check_range(sim, first, count);
+    if (st) return st;
+    if (count > 0) sim->eng.force_range(0, (int)first, (int)(first + count));
+    sim->have_acc = true;
+    return NB_OK;
+}
+
+nb_status nb_stage_brute_force_range(nb_sim* sim, int64_t first, int64_t count) {
+    if (!sim) return fail(NB_E_INVALID_ARG, "NULL sim");
+    nb_status st = need(sim->have_state && !is_bh(sim), "brute force needs uploaded state and a brute-force algorithm");
+    if (!st) st = check_range(sim, first, count);
+    if (st) return st;
+    if (count > 0) sim->eng.brute_range(0, (int)first, (int)(first + count));
+    sim->have_acc = true;
+    return NB_OK;
+}
+
 nb_status nb_stage_collision(nb_sim* sim) {
     if (!sim) return fail(NB_E_INVALID_ARG, "NULL sim");
     nb_status st = need(sim->have_state, "collision needs uploaded state");

@@ -34,6 +34,8 @@ void launch_force_bh(hipStream_t s, int nblk, bool kahan, const SysDesc* sd, con
                      const int* sorted, float4* acc4);
 void launch_brute(hipStream_t s, int nblk, int law, bool kahan, int tb, const SysDesc* sd, const SysState* st,
                   const BlkWork* work, const float4* node4, float4* acc4);
+void launch_range_table(hipStream_t s, BlkWork* out, int nblk, int sys, int start, int stop, int lo, int hi,
+                        int per);
 void launch_collision(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
                       const float4* node4, float4* vel4);
 void launch_vertices(hipStream_t s, int n, const float4* node4, float4* out);
@@ -105,8 +107,9 @@ struct Engine {
     int nactive = 0;
 
     // block tables (device) with their capacities and current lengths
+    // T_RANGE is filled on the device by each target-range force launch
     enum Table { T_BOUNDS, T_BODY, T_CELLS, T_GROUPS, T_BRUTE, T_UPDATE, T_PHASE0, T_PHASE1, T_PHASE2, T_PHASE3,
-                 T_COUNT };
+                 T_RANGE, T_COUNT };
     BlkWork* tab[T_COUNT] = {};
     int tab_cap[T_COUNT] = {};
     int tab_len[T_COUNT] = {};
@@ -131,6 +134,11 @@ struct Engine {
     void sort();
     void force();
     void brute();
+    // One system's force pass over the targets [lo, hi), 0 <= lo < hi <= n: tree
+    // order for Barnes-Hut (vote groups stay those of the full pass), body index
+    // for brute force.  Results equal the full pass's for those targets.
+    void force_range(int sys, int lo, int hi);
+    void brute_range(int sys, int lo, int hi);
     void update(int mode);
     void collide();
 

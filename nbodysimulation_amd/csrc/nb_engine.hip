@@ -130,6 +130,7 @@ nb_status Engine::reserve(const Capacity& c, int law_, bool kahan_, void* user_s
     tab_cap[T_GROUPS] = blocks_bound(c.bodies, kForceVote * kForceWaves, S);
     tab_cap[T_BRUTE] = blocks_bound(c.bodies, 16, S);
     tab_cap[T_UPDATE] = blocks_bound(c.bodies, 256, S);
+    tab_cap[T_RANGE] = blocks_bound(c.bodies, 16, S);  // the smallest share per block of either force kernel
     for (int ph = 0; ph < kBuildPhases; ++ph)
         tab_cap[T_PHASE0 + ph] = blocks_bound((c.bodies + kPhaseStride[ph] - 1) / kPhaseStride[ph], kBuildBlock, S);
 
@@ -327,6 +328,21 @@ void Engine::sort() {
 void Engine::force() {
     launch_force_bh(stream, tab_len[T_GROUPS], kahan, d_desc, d_state, tab[T_GROUPS], node4, child, cdata, sorted,
                     acc4);
+}
+void Engine::force_range(int sys, int lo, int hi) {
+    const int per = kForceVote * kForceWaves;
+    const int start = lo / kForceVote * kForceVote;
+    const int stop = (int)std::min<long long>(hdesc[sys].n, ((long long)hi + kForceVote - 1) / kForceVote * kForceVote);
+    const int nblk = (stop - start + per - 1) / per;
+    launch_range_table(stream, tab[T_RANGE], nblk, sys, start, stop, lo, hi, per);
+    launch_force_bh(stream, nblk, kahan, d_desc, d_state, tab[T_RANGE], node4, child, cdata, sorted, acc4);
+}
+void Engine::brute_range(int sys, int lo, int hi) {
+    const SysDesc& d = hdesc[sys];
+    const int per = (kBfWaves >> d.bf_split_log2) * (64 >> d.bf_lanes_log2) * bf_tb;
+    const int nblk = (hi - lo + per - 1) / per;
+    launch_range_table(stream, tab[T_RANGE], nblk, sys, lo, hi, lo, hi, per);
+    launch_brute(stream, nblk, law, kahan, bf_tb, d_desc, d_state, tab[T_RANGE], node4, acc4);
 }
 void Engine::brute() {
     launch_brute(stream, tab_len[T_BRUTE], law, kahan, bf_tb, d_desc, d_state, tab[T_BRUTE], node4, acc4);

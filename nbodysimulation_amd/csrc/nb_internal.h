@@ -71,11 +71,15 @@ struct SysState {
     int pad1[3];
 };
 
+// One workgroup's share of one system: items [begin, end).  `trim` is only set
+// by target-range force launches: its low and high 16 bits count the leading and
+// trailing items of the share that take part in the work (a vote group keeps all
+// its members) but whose results are not written.
 struct BlkWork {
     int sys;
     int begin;
     int end;
-    int pad;
+    int trim;
 };
 
 }  // namespace nb

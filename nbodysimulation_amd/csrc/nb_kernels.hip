@@ -12,6 +12,7 @@
 //   k_update      integrator branch                          kernels/calculate_forces.cl:150-182
 //   k_collision   update_collision_local                     kernels/collision_update.cl:3-83
 //   k_vertices    copy_vertices                              kernels/copy_vertices.cl:3-17
+//   k_range_table block table of one target range (no counterpart: one GPU's share of a force pass)
 //   k_energy_f64, k_energy_final   FP64 conservation diagnostics (no counterpart)
 //
 // Memory-model notes (gfx950 has one L2 per XCD):
@@ -640,10 +641,22 @@ __global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __
             }
         }
     }
-    if (valid) {
+    // a target-range launch votes with whole groups but writes only its range
+    if (valid && idx >= w.begin + (w.trim & 0xffff) && idx < w.end - (w.trim >> 16)) {
         if (KAHAN) { ax -= kx; ay -= ky; az -= kz; }
         acc4[d.body_off + b] = make_float4(ax * d.G, ay * d.G, az * d.G, (float)nint);
     }
+}
+
+// Block table of one target range [lo, hi) of system `sys`: blocks of `per` items
+// from `start` (<= lo) to `stop` (>= hi); trim marks the items outside [lo, hi).
+__global__ __launch_bounds__(256) void k_range_table(BlkWork* __restrict__ out, int nblk, int sys, int start,
+                                                     int stop, int lo, int hi, int per) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nblk) return;
+    const int b = start + i * per, e = min(stop, b + per);
+    const int head = min(max(lo - b, 0), e - b), tail = min(max(e - hi, 0), e - b);
+    out[i] = BlkWork{sys, b, e, head | (tail << 16)};
 }
 
 // ---------------------------------------------------------------------------
@@ -1022,6 +1035,10 @@ void launch_force_bh(hipStream_t s, int nblk, bool kahan, const SysDesc* sd, con
         NB_LAUNCH(k_force_bh<true>, nblk, 64 * kForceWaves, s, sd, st, work, node4, child, cdata, sorted, acc4);
     else
         NB_LAUNCH(k_force_bh<false>, nblk, 64 * kForceWaves, s, sd, st, work, node4, child, cdata, sorted, acc4);
+}
+void launch_range_table(hipStream_t s, BlkWork* out, int nblk, int sys, int start, int stop, int lo, int hi,
+                        int per) {
+    NB_LAUNCH(k_range_table, (nblk + 255) / 256, 256, s, out, nblk, sys, start, stop, lo, hi, per);
 }
 void launch_brute(hipStream_t s, int nblk, int law, bool kahan, int tb, const SysDesc* sd, const SysState* st,
                   const BlkWork* work, const float4* node4, float4* acc4) {

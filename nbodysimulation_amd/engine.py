@@ -104,6 +104,14 @@ class NBodySim:
     def force(self): self._call("nb_stage_force")
     def brute_force(self): self._call("nb_stage_brute_force")
     def collision(self): self._call("nb_stage_collision")
+
+    def force_range(self, first: int, count: int):
+        """Barnes-Hut force for the targets [first, first+count) of the tree order only."""
+        self._call("nb_stage_force_range", int(first), int(count))
+
+    def brute_force_range(self, first: int, count: int):
+        """Brute-force force for the bodies [first, first+count) only."""
+        self._call("nb_stage_brute_force_range", int(first), int(count))
     def update(self): self._call("nb_stage_update")
 
     def force_pipeline(self):
