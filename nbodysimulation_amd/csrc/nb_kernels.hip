@@ -662,7 +662,10 @@ __global__ __launch_bounds__(256) void k_range_table(BlkWork* __restrict__ out, 
 // ---------------------------------------------------------------------------
 // brute force: all pairs (brute_force_update.cl:3-105 `update`, :108-228 `update_local`).
 // LAW 0: d*m*rsqrt(|d|^2+eps^2)^3.  LAW 1 (brute_force_update.cl:54-58,114-120):
-// n = normalize(d), n*m/(|n|^2+ueps^2).
+// n = normalize(d), n*m/(|n|^2+ueps^2).  A normalised n has |n|^2 = 1 (the
+// reference's recomputed |n|^2 differs from 1 by rounding only), so LAW 1 is
+// evaluated as d * rsqrt(|d|^2) * (m/(1+ueps^2)), with m/(1+ueps^2) applied
+// once per source when it is staged.
 //
 // Workgroup = 16 waves.  A wave holds 64*TB/W targets: W lanes per target
 // (W = 2^bf_lanes_log2) and TB targets per lane (register blocking).  S =
@@ -689,11 +692,11 @@ __device__ __forceinline__ void bf_pair(const float4 q, float px, float py, floa
         s = q.w * ri3;
         fx = dx; fy = dy; fz = dz;
     } else {
+        // q.w = m/(1+ueps^2).  A zero displacement (the self pair) is clamped to the
+        // smallest normal |d|^2: rsqrt stays finite and d * rsqrt = 0
         const float l2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-        const float ri = l2 > 0.0f ? __builtin_amdgcn_rsqf(l2) : 0.0f;
-        fx = dx * ri; fy = dy * ri; fz = dz * ri;
-        const float d2 = fmaf(fz, fz, fmaf(fy, fy, fmaf(fx, fx, e2)));
-        s = q.w * __builtin_amdgcn_rcpf(d2);
+        s = q.w * __builtin_amdgcn_rsqf(fmaxf(l2, 1.17549435e-38f));
+        fx = dx; fy = dy; fz = dz;
     }
 }
 
@@ -722,6 +725,7 @@ __device__ __forceinline__ void brute_body(const BlkWork& w, const SysDesc& d, c
     const float4* __restrict__ nodes = node4 + d.node_off;
     const float e = LAW == 0 ? d.eps : d.ueps;
     const float e2 = e * e;
+    const float cu = 1.0f / (1.0f + e2);  // LAW 1: 1/(|n|^2 + ueps^2) with |n|^2 = 1
     const int gbase = w.begin + grp * L * TB;
     float px[TB], py[TB], pz[TB], acc[NC][TB];
 #pragma unroll
@@ -740,7 +744,11 @@ __device__ __forceinline__ void brute_body(const BlkWork& w, const SysDesc& d, c
     for (int jt = j0; jt < j1; jt += 64) {
         const int cnt = min(64, j1 - jt);
         wave_sync();
-        if (lane < cnt) tw[store_at] = nodes[jt + lane];
+        if (lane < cnt) {
+            float4 q = nodes[jt + lane];
+            if (LAW == 1) q.w *= cu;
+            tw[store_at] = q;
+        }
         wave_sync();
         const int my = min(max(cnt - pw * L, 0), L);
 #pragma unroll 4
