@@ -30,8 +30,11 @@ struct nb_sim {
     bool have_summary = false;
     bool have_sorted = false;
     bool have_acc = false;
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph = nullptr;  // one step
     hipGraph_t graph_def = nullptr;
+    hipGraphExec_t fgraph[2] = {};    // fused brute-force steps: one step, two steps
+    hipGraph_t fgraph_def[2] = {};
+    float4* pos_b = nullptr;          // second position buffer of fused Euler steps
     std::vector<hipEvent_t> ev;  // kEvPerStep per timed step
     float stage_ms[NB_NUM_STAGES] = {};
     double* energy_buf = nullptr;  // per-workgroup partial sums + result of nb_sim_energy
@@ -160,6 +163,12 @@ static void drop_graph(nb_sim* sim) {
     if (sim->graph_def) (void)hipGraphDestroy(sim->graph_def);
     sim->graph = nullptr;
     sim->graph_def = nullptr;
+    for (int k = 0; k < 2; ++k) {
+        if (sim->fgraph[k]) (void)hipGraphExecDestroy(sim->fgraph[k]);
+        if (sim->fgraph_def[k]) (void)hipGraphDestroy(sim->fgraph_def[k]);
+        sim->fgraph[k] = nullptr;
+        sim->fgraph_def[k] = nullptr;
+    }
 }
 
 nb_status nb_sim_create(const nb_config* cfg, void* stream, nb_sim** out) {
@@ -194,6 +203,7 @@ nb_status nb_sim_destroy(nb_sim* sim) {
     for (auto& e : sim->ev)
         if (e) (void)hipEventDestroy(e);
     if (sim->energy_buf) (void)hipFree(sim->energy_buf);
+    if (sim->pos_b) (void)hipFree(sim->pos_b);
     delete sim;
     return NB_OK;
 }
@@ -474,10 +484,42 @@ static nb_status read_stage_times(nb_sim* sim, int nsteps) {
     return NB_OK;
 }
 
-static nb_status build_graph(nb_sim* sim) {
+// Untimed brute-force steps of a small system without collision fuse the
+// integrator into the brute-force kernel (one launch per Euler step instead of
+// two).  Euler moves positions, which the kernel still reads as sources, so
+// fused Euler steps alternate between node4 and pos_b: two steps return to
+// node4, a single step copies back.  Leapfrog's opening kick and drift stay a
+// launch of their own; the closing kick is fused.  Results are bit-identical to
+// the separate launches.
+static bool fusable(const nb_sim* sim) {
+    return !is_bh(sim) && !sim->cfg.collision && sim->n > 0 && sim->n <= nb::kBfFuseMax;
+}
+
+static void enqueue_fused(nb_sim* sim, int k) {
+    Engine& g = sim->eng;
+    if (sim->cfg.method == NB_LEAPFROG) {
+        for (int i = 0; i < k; ++i) {
+            g.update(2);
+            g.brute_step(1, g.node4, nullptr);
+        }
+        return;
+    }
+    g.brute_step(0, g.node4, sim->pos_b);
+    if (k == 2)
+        g.brute_step(0, sim->pos_b, g.node4);
+    else
+        (void)hipMemcpyAsync(g.node4, sim->pos_b, sizeof(float4) * (size_t)sim->n, hipMemcpyDeviceToDevice,
+                             g.stream);
+}
+
+// k = 0: one plain step; k = 1, 2: that many fused steps
+static nb_status build_graph(nb_sim* sim, int k, hipGraph_t* def, hipGraphExec_t* exec) {
     Engine& e = sim->eng;
     NB_HIPC(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
-    enqueue_step(sim, nullptr);
+    if (k == 0)
+        enqueue_step(sim, nullptr);
+    else
+        enqueue_fused(sim, k);
     hipGraph_t g = nullptr;
     hipError_t ce = hipStreamEndCapture(e.stream, &g);
     if (ce != hipSuccess) return fail(NB_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
@@ -487,9 +529,47 @@ static nb_status build_graph(nb_sim* sim) {
         (void)hipGraphDestroy(g);
         return fail(NB_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(ce));
     }
-    sim->graph_def = g;
-    sim->graph = ex;
+    *def = g;
+    *exec = ex;
     return NB_OK;
+}
+
+// nsteps untimed steps on the stream (no synchronisation)
+static nb_status enqueue_steps(nb_sim* sim, int nsteps) {
+    Engine& e = sim->eng;
+    if (fusable(sim)) {
+        if (sim->cfg.method == NB_EULER && !sim->pos_b) NB_HIPC(hipMalloc(&sim->pos_b, sizeof(float4) * sim->n));
+        if (sim->cfg.use_graph) {
+            for (int k = 1; k <= 2; ++k)
+                if (!sim->fgraph[k - 1]) {
+                    nb_status st = build_graph(sim, k, &sim->fgraph_def[k - 1], &sim->fgraph[k - 1]);
+                    if (st) return st;
+                }
+            for (int i = 0; i + 1 < nsteps; i += 2) NB_HIPC(hipGraphLaunch(sim->fgraph[1], e.stream));
+            if (nsteps & 1) NB_HIPC(hipGraphLaunch(sim->fgraph[0], e.stream));
+        } else {
+            for (int i = 0; i + 1 < nsteps; i += 2) enqueue_fused(sim, 2);
+            if (nsteps & 1) enqueue_fused(sim, 1);
+        }
+    } else if (sim->cfg.use_graph) {
+        if (!sim->graph) {
+            nb_status st = build_graph(sim, 0, &sim->graph_def, &sim->graph);
+            if (st) return st;
+        }
+        for (int i = 0; i < nsteps; ++i) NB_HIPC(hipGraphLaunch(sim->graph, e.stream));
+    } else {
+        for (int i = 0; i < nsteps; ++i) enqueue_step(sim, nullptr);
+    }
+    NB_HIPC(hipGetLastError());
+    return NB_OK;
+}
+
+static void after_steps(nb_sim* sim, int nsteps) {
+    sim->have_box = false;
+    sim->tree_clean = false;
+    sim->have_tree = sim->have_summary = sim->have_sorted = is_bh(sim);
+    sim->have_acc = true;
+    sim->steps += nsteps;
 }
 
 nb_status nb_step(nb_sim* sim, int32_t nsteps) {
@@ -505,28 +585,18 @@ nb_status nb_step(nb_sim* sim, int32_t nsteps) {
     if (timed) {
         nb_status st = ensure_events(sim, (size_t)nsteps);
         if (st) return st;
-    }
-    int done = 0;
-    if (sim->cfg.use_graph && !timed) {
-        if (!sim->graph) {
-            nb_status st = build_graph(sim);
-            if (st) return st;
-        }
-        for (; done < nsteps; ++done) NB_HIPC(hipGraphLaunch(sim->graph, e.stream));
+        for (int i = 0; i < nsteps; ++i) enqueue_step(sim, sim->ev.data() + kEvPerStep * (size_t)i);
+        NB_HIPC(hipGetLastError());
     } else {
-        for (; done < nsteps; ++done) enqueue_step(sim, timed ? sim->ev.data() + kEvPerStep * (size_t)done : nullptr);
+        nb_status st = enqueue_steps(sim, nsteps);
+        if (st) return st;
     }
-    NB_HIPC(hipGetLastError());
     NB_HIPC(hipStreamSynchronize(e.stream));
     if (timed) {
         nb_status st = read_stage_times(sim, nsteps);
         if (st) return st;
     }
-    sim->have_box = false;
-    sim->tree_clean = false;
-    sim->have_tree = sim->have_summary = sim->have_sorted = is_bh(sim);
-    sim->have_acc = true;
-    sim->steps += nsteps;
+    after_steps(sim, nsteps);
     return nb_sim_status(sim);
 }
 
@@ -534,26 +604,13 @@ nb_status nb_step_async(nb_sim* sim, int32_t nsteps) {
     if (!sim || nsteps < 0) return fail(NB_E_INVALID_ARG, "bad arguments");
     if (!sim->have_state) return fail(NB_E_STATE, "nb_step_async needs uploaded state");
     if (nsteps == 0 || sim->n == 0) return NB_OK;
-    Engine& e = sim->eng;
     if (sim->cfg.method == NB_LEAPFROG && !sim->have_acc) {
         enqueue_force(sim, nullptr);
         sim->have_acc = true;
     }
-    if (sim->cfg.use_graph) {
-        if (!sim->graph) {
-            nb_status st = build_graph(sim);
-            if (st) return st;
-        }
-        for (int i = 0; i < nsteps; ++i) NB_HIPC(hipGraphLaunch(sim->graph, e.stream));
-    } else {
-        for (int i = 0; i < nsteps; ++i) enqueue_step(sim, nullptr);
-    }
-    NB_HIPC(hipGetLastError());
-    sim->have_box = false;
-    sim->tree_clean = false;
-    sim->have_tree = sim->have_summary = sim->have_sorted = is_bh(sim);
-    sim->have_acc = true;
-    sim->steps += nsteps;
+    nb_status st = enqueue_steps(sim, nsteps);
+    if (st) return st;
+    after_steps(sim, nsteps);
     return NB_OK;
 }
 

@@ -32,8 +32,10 @@ void launch_sort(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const
 void launch_force_bh(hipStream_t s, int nblk, bool kahan, const SysDesc* sd, const SysState* st,
                      const BlkWork* work, const float4* node4, const int* child, const float4* cdata,
                      const int* sorted, float4* acc4);
+// mode >= 0: also integrate (k_update's mode 0 or 1); mode 0 writes the moved
+// positions to out4 (same layout as node4), never in place
 void launch_brute(hipStream_t s, int nblk, int law, bool kahan, int tb, const SysDesc* sd, const SysState* st,
-                  const BlkWork* work, const float4* node4, float4* acc4);
+                  const BlkWork* work, const float4* node4, float4* acc4, float4* vel4, float4* out4, int mode);
 void launch_range_table(hipStream_t s, BlkWork* out, int nblk, int sys, int start, int stop, int lo, int hi,
                         int per);
 void launch_collision(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
@@ -134,6 +136,10 @@ struct Engine {
     void sort();
     void force();
     void brute();
+    // Brute force with the integrator of `mode` (0 Euler, 1 closing kick) fused:
+    // positions are read from `in` (node4 layout) and, for mode 0, the moved ones
+    // written to `out`; velocities are updated in place.
+    void brute_step(int mode, const float4* in, float4* out);
     // One system's force pass over the targets [lo, hi), 0 <= lo < hi <= n: tree
     // order for Barnes-Hut (vote groups stay those of the full pass), body index
     // for brute force.  Results equal the full pass's for those targets.

@@ -342,10 +342,15 @@ void Engine::brute_range(int sys, int lo, int hi) {
     const int per = (kBfWaves >> d.bf_split_log2) * (64 >> d.bf_lanes_log2) * bf_tb;
     const int nblk = (hi - lo + per - 1) / per;
     launch_range_table(stream, tab[T_RANGE], nblk, sys, lo, hi, lo, hi, per);
-    launch_brute(stream, nblk, law, kahan, bf_tb, d_desc, d_state, tab[T_RANGE], node4, acc4);
+    launch_brute(stream, nblk, law, kahan, bf_tb, d_desc, d_state, tab[T_RANGE], node4, acc4, vel4, nullptr, -1);
 }
 void Engine::brute() {
-    launch_brute(stream, tab_len[T_BRUTE], law, kahan, bf_tb, d_desc, d_state, tab[T_BRUTE], node4, acc4);
+    launch_brute(stream, tab_len[T_BRUTE], law, kahan, bf_tb, d_desc, d_state, tab[T_BRUTE], node4, acc4, vel4,
+                 nullptr, -1);
+}
+void Engine::brute_step(int mode, const float4* in, float4* out) {
+    launch_brute(stream, tab_len[T_BRUTE], law, kahan, bf_tb, d_desc, d_state, tab[T_BRUTE], in, acc4, vel4, out,
+                 mode);
 }
 void Engine::update(int mode) {
     launch_update(stream, tab_len[T_UPDATE], d_desc, d_state, tab[T_UPDATE], node4, vel4, acc4, mode);

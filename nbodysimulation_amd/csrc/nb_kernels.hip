@@ -660,6 +660,23 @@ __global__ __launch_bounds__(256) void k_range_table(BlkWork* __restrict__ out, 
 }
 
 // ---------------------------------------------------------------------------
+// integrator arithmetic of one body (calculate_forces.cl:150-182), shared by
+// k_update and the brute-force kernel's fused form: a kick v += a*h, a drift
+// x += v*dt (w, the mass, is kept).
+__device__ __forceinline__ float4 kick(float4 v, float ax, float ay, float az, float h) {
+    v.x = fmaf(ax, h, v.x);
+    v.y = fmaf(ay, h, v.y);
+    v.z = fmaf(az, h, v.z);
+    return v;
+}
+__device__ __forceinline__ float4 drift(float4 x, float4 v, float dt) {
+    x.x = fmaf(v.x, dt, x.x);
+    x.y = fmaf(v.y, dt, x.y);
+    x.z = fmaf(v.z, dt, x.z);
+    return x;
+}
+
+// ---------------------------------------------------------------------------
 // brute force: all pairs (brute_force_update.cl:3-105 `update`, :108-228 `update_local`).
 // LAW 0: d*m*rsqrt(|d|^2+eps^2)^3.  LAW 1 (brute_force_update.cl:54-58,114-120):
 // n = normalize(d), n*m/(|n|^2+ueps^2).  A normalised n has |n|^2 = 1 (the
@@ -712,7 +729,8 @@ __device__ __forceinline__ void kahan_add(float& a, float& c, float v) {
 template <int LAW, bool KAHAN, int TB, int W>
 __device__ __forceinline__ void brute_body(const BlkWork& w, const SysDesc& d, const float4* __restrict__ node4,
                                            float4* __restrict__ acc4, float4 (*tile)[kBfTileStride],
-                                           float (*red)[2 * 6][64]) {
+                                           float (*red)[2 * 6][64], float4* __restrict__ vel4,
+                                           float4* __restrict__ out4, int mode) {
     constexpr int NC = KAHAN ? 6 : 3;
     constexpr int L = 64 / W;  // lanes (targets) per part, sources per part and tile
     // the wave index is uniform over the wave: keep it (and the loop bounds) scalar
@@ -727,12 +745,12 @@ __device__ __forceinline__ void brute_body(const BlkWork& w, const SysDesc& d, c
     const float e2 = e * e;
     const float cu = 1.0f / (1.0f + e2);  // LAW 1: 1/(|n|^2 + ueps^2) with |n|^2 = 1
     const int gbase = w.begin + grp * L * TB;
-    float px[TB], py[TB], pz[TB], acc[NC][TB];
+    float px[TB], py[TB], pz[TB], pm[TB], acc[NC][TB];
 #pragma unroll
     for (int t = 0; t < TB; ++t) {
         const int i = min(gbase + t * L + tl, w.end - 1);
         const float4 p = nodes[i];
-        px[t] = p.x; py[t] = p.y; pz[t] = p.z;
+        px[t] = p.x; py[t] = p.y; pz[t] = p.z; pm[t] = p.w;
 #pragma unroll
         for (int c = 0; c < NC; ++c) acc[c][t] = 0.0f;
     }
@@ -826,24 +844,36 @@ __device__ __forceinline__ void brute_body(const BlkWork& w, const SysDesc& d, c
         if (i < w.end) {
             float ax = acc[0][t], ay = acc[1][t], az = acc[2][t];
             if (KAHAN) { ax -= acc[3][t]; ay -= acc[4][t]; az -= acc[5][t]; }
-            acc4[d.body_off + i] = make_float4(ax * d.G, ay * d.G, az * d.G, (float)(n - 1));
+            const float4 a = make_float4(ax * d.G, ay * d.G, az * d.G, (float)(n - 1));
+            acc4[d.body_off + i] = a;
+            if (mode >= 0) {  // fused integrator, k_update's arithmetic
+                const float4 v = kick(vel4[d.body_off + i], a.x, a.y, a.z, mode == 0 ? d.dt : 0.5f * d.dt);
+                vel4[d.body_off + i] = v;
+                if (mode == 0) out4[d.node_off + i] = drift(make_float4(px[t], py[t], pz[t], pm[t]), v, d.dt);
+            }
         }
     }
 }
 
+// mode >= 0: the integrator of that mode (k_update's) follows in the lane that
+// writes each target's acceleration.  Velocities are read by no other lane, so
+// they are updated in place; mode 0 also moves the positions, which every
+// workgroup still reads as sources, so the moved positions (with the masses) go
+// to out4, a second position buffer, and the caller alternates the two.
 template <int LAW, bool KAHAN, int TB>
 __global__ __launch_bounds__(1024) void k_brute(const SysDesc* __restrict__ sd, const SysState* __restrict__ st,
-                                                const BlkWork* __restrict__ work,
-                                                const float4* __restrict__ node4, float4* __restrict__ acc4) {
+                                                const BlkWork* __restrict__ work, const float4* __restrict__ node4,
+                                                float4* __restrict__ acc4, float4* __restrict__ vel4,
+                                                float4* __restrict__ out4, int mode) {
     __shared__ float4 tile[kBfWaves][kBfTileStride];
     __shared__ float red[kBfWaves][2 * 6][64];
     const BlkWork w = work[blockIdx.x];
     const SysDesc d = sd[w.sys];
     if (st[w.sys].status) return;  // whole workgroup
     switch (d.bf_lanes_log2) {     // uniform over the workgroup
-    case 0: brute_body<LAW, KAHAN, TB, 1>(w, d, node4, acc4, tile, red); break;
-    case 1: brute_body<LAW, KAHAN, TB, 2>(w, d, node4, acc4, tile, red); break;
-    default: brute_body<LAW, KAHAN, TB, 4>(w, d, node4, acc4, tile, red); break;
+    case 0: brute_body<LAW, KAHAN, TB, 1>(w, d, node4, acc4, tile, red, vel4, out4, mode); break;
+    case 1: brute_body<LAW, KAHAN, TB, 2>(w, d, node4, acc4, tile, red, vel4, out4, mode); break;
+    default: brute_body<LAW, KAHAN, TB, 4>(w, d, node4, acc4, tile, red, vel4, out4, mode); break;
     }
 }
 
@@ -988,19 +1018,9 @@ __global__ __launch_bounds__(256) void k_update(const SysDesc* __restrict__ sd, 
     const int i = w.begin + threadIdx.x;
     if (i >= w.end) return;
     const float4 a = acc4[d.body_off + i];
-    float4 v = vel4[d.body_off + i];
-    float4 x = node4[d.node_off + i];
-    const float h = mode == 0 ? d.dt : 0.5f * d.dt;
-    v.x = fmaf(a.x, h, v.x);
-    v.y = fmaf(a.y, h, v.y);
-    v.z = fmaf(a.z, h, v.z);
+    const float4 v = kick(vel4[d.body_off + i], a.x, a.y, a.z, mode == 0 ? d.dt : 0.5f * d.dt);
     vel4[d.body_off + i] = v;
-    if (mode != 1) {
-        x.x = fmaf(v.x, d.dt, x.x);
-        x.y = fmaf(v.y, d.dt, x.y);
-        x.z = fmaf(v.z, d.dt, x.z);
-        node4[d.node_off + i] = x;
-    }
+    if (mode != 1) node4[d.node_off + i] = drift(node4[d.node_off + i], v, d.dt);
 }
 
 // ---------------------------------------------------------------------------
@@ -1049,10 +1069,11 @@ void launch_range_table(hipStream_t s, BlkWork* out, int nblk, int sys, int star
     NB_LAUNCH(k_range_table, (nblk + 255) / 256, 256, s, out, nblk, sys, start, stop, lo, hi, per);
 }
 void launch_brute(hipStream_t s, int nblk, int law, bool kahan, int tb, const SysDesc* sd, const SysState* st,
-                  const BlkWork* work, const float4* node4, float4* acc4) {
+                  const BlkWork* work, const float4* node4, float4* acc4, float4* vel4, float4* out4, int mode) {
     const int B = kBfWaves * 64;
-#define NB_BF(L, K, T) \
-    if (law == L && kahan == K && tb == T) NB_LAUNCH((k_brute<L, K, T>), nblk, B, s, sd, st, work, node4, acc4)
+#define NB_BF(L, K, T)                                                                                   \
+    if (law == L && kahan == K && tb == T)                                                               \
+    NB_LAUNCH((k_brute<L, K, T>), nblk, B, s, sd, st, work, node4, acc4, vel4, out4, mode)
     NB_BF(0, false, 1); NB_BF(0, true, 1); NB_BF(1, false, 1); NB_BF(1, true, 1);
     NB_BF(0, false, 2); NB_BF(0, true, 2); NB_BF(1, false, 2); NB_BF(1, true, 2);
 #undef NB_BF

@@ -1,0 +1,94 @@
+"""Untimed brute-force steps of systems up to 16384 bodies (no collision stage)
+run the integrator inside the brute-force kernel: Euler steps alternate between
+two position buffers (the kernel still reads the old positions as sources), and
+leapfrog's closing kick follows each acceleration.  The fused steps must equal
+the separate stage launches (brute force, then update) bit for bit, for every
+method, law and step count, at sizes on both sides of the 16384-body limit."""
+import pytest
+
+from nbodysimulation_amd import _lib as L
+from nbodysimulation_amd import ic
+from nbodysimulation_amd.engine import NBodyEnsemble, NBodySim, make_config, system_spec
+
+pytestmark = pytest.mark.gpu
+
+FUSE_MAX = 16384
+
+
+def stage_calls(cfg, state, steps):
+    with NBodySim(cfg) as s:
+        s.upload(*state)
+        for _ in range(steps):
+            s.brute_force()
+            s.update()
+        return s.download()
+
+
+@pytest.mark.parametrize("n", [1, 63, 4096, FUSE_MAX, FUSE_MAX + 1])
+@pytest.mark.parametrize("law", [L.LAW_REFERENCE_UNIT, L.LAW_PLUMMER])
+@pytest.mark.parametrize("algo", [L.BRUTE_FORCE_GLOBAL, L.BRUTE_FORCE_LOCAL])
+def test_fused_euler_equals_stage_calls(gpu, n, law, algo):
+    print(f"case n={n} law={law} algo={algo}", flush=True)
+    state = ic.normal_cloud(n, 50 + n)
+    cfg = make_config(n, algo, law=law, eps=0.01, G=1.0, dt=1e-3)
+    with NBodySim(cfg) as s:
+        s.upload(*state)
+        s.step(7)  # three two-step replays and one single step
+        fused = s.download()
+        assert s.status() == L.NB_OK
+    for a, b in zip(fused, stage_calls(cfg, state, 7)):
+        assert a.tobytes() == b.tobytes()
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_fused_step_counts(gpu, use_graph):
+    """Any split of the same steps into calls (1, 2, odd, even; graph or stream
+    launches) gives the same state as the stage calls."""
+    n = 2000
+    state = ic.plummer(n, 70)
+    cfg = make_config(n, L.BRUTE_FORCE_GLOBAL, law=L.LAW_PLUMMER, use_graph=use_graph)
+    with NBodySim(cfg) as s:
+        s.upload(*state)
+        for k in (1, 2, 3, 1, 4, 1):
+            s.step(k)
+        got = s.download()
+        assert s.step_count() == 12
+    for a, b in zip(got, stage_calls(cfg, state, 12)):
+        assert a.tobytes() == b.tobytes()
+
+
+@pytest.mark.parametrize("n", [100, 4096])
+def test_fused_leapfrog_equals_timed_steps(gpu, n):
+    """Timed steps launch every stage on its own (each is timed): they must equal the
+    fused replay bit for bit for kick-drift-kick too."""
+    state = ic.plummer(n, 60)
+    outs = []
+    for timed in (False, True):
+        cfg = make_config(n, L.BRUTE_FORCE_GLOBAL, method=L.LEAPFROG, law=L.LAW_PLUMMER, time_stages=timed)
+        with NBodySim(cfg) as s:
+            s.upload(*state)
+            s.step(5)
+            s.step(6)
+            outs.append(s.download())
+    for a, b in zip(*outs):
+        assert a.tobytes() == b.tobytes()
+
+
+def test_fused_alone_equals_batched(gpu):
+    """A fused single system equals the same system in an ensemble (whose steps are
+    never fused) after a thousand steps."""
+    n = 3000
+    state = ic.uniform_ball(n, 61)
+    cfg = make_config(n, L.BRUTE_FORCE_GLOBAL, law=L.LAW_PLUMMER, G=1.0, eps=0.01, dt=1e-3)
+    with NBodySim(cfg) as s:
+        s.upload(*state)
+        s.step(1000)
+        alone = s.download()
+    with NBodyEnsemble([system_spec(n, L.BRUTE_FORCE_GLOBAL), system_spec(5000, L.BARNES_HUT)],
+                       law=L.LAW_PLUMMER) as ens:
+        ens.upload(0, *state)
+        ens.upload(1, *ic.plummer(5000, 62))
+        ens.step(1000)
+        batched = ens.download(0)
+    for a, b in zip(alone, batched):
+        assert a.tobytes() == b.tobytes()
