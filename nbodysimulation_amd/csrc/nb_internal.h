@@ -67,7 +67,7 @@ struct SysState {
     int max_depth;
     int ticket;     // bounds last-block ticket (self-resetting)
     int step;
-    int pad0;
+    int bq;         // next entry of the build's last-phase body queue
     float dq[kDqLen];
     int pad1[3];
 };

@@ -256,6 +256,7 @@ __global__ void k_tree_init(int nsys, const int* __restrict__ slots, SysState* s
     const int s = slots ? slots[k] : k;
     st[s].ncells = 1;
     st[s].max_depth = 1;  // build_tree.cl:28
+    st[s].bq = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -304,29 +305,65 @@ __global__ __launch_bounds__(kBuildBlock) void k_build(const SysDesc* __restrict
     if (S->status) return;
     const int n = d.n;
     const int lane = threadIdx.x & 63;
-    const int bi = w.begin + threadIdx.x;  // index in this phase's strided subset
-    const int b = bi * stride;
-    bool active = bi < w.end && b < n && (exclude == 0 || b % exclude != 0);
+    const int wave = threadIdx.x >> 6;
+    // The last phase (stride 1) takes its bodies from a per-system queue: a lane whose
+    // insertion is complete claims the next body at the round's allocation, so that no
+    // lane idles while others of its workgroup retry under contention.  Queue entry j
+    // is the j-th body that the earlier phases left out (not a multiple of `exclude`).
+    const bool queued = stride == 1;
+    const int nq = exclude > 1 ? n - (n + exclude - 1) / exclude : (exclude == 1 ? 0 : n);
     int* C = child + 8 * (size_t)d.cell_off;
     int* par = cparent + d.cell_off;
     int* pend = cpend + d.cell_off;
     float rlx, rly, rlz, rhx, rhy, rhz;
     root_box(d.rules, S->box, rlx, rly, rlz, rhx, rhy, rhz);
     float px = 0.f, py = 0.f, pz = 0.f;
-    if (active) {
-        const float4 p = node4[d.node_off + b];
-        px = p.x; py = p.y; pz = p.z;
-    }
     float lx = rlx, ly = rly, lz = rlz, hx = rhx, hy = rhy, hz = rhz;
-    int node = 0, lvl = 0;
-    int dg = level_digit(d.rules, 0, px, py, pz, lx, ly, lz, hx, hy, hz);
+    int b = 0, node = 0, lvl = 0, dg = 0;
     int depth = 1;
     int tries = 0;
     bool fresh = false;  // re-read the current slot coherently
+    bool active = false;
+    auto start = [&](int body) {
+        b = body;
+        const float4 p = node4[d.node_off + b];
+        px = p.x; py = p.y; pz = p.z;
+        lx = rlx; ly = rly; lz = rlz; hx = rhx; hy = rhy; hz = rhz;
+        node = 0;
+        lvl = 0;
+        dg = level_digit(d.rules, 0, px, py, pz, lx, ly, lz, hx, hy, hz);
+        tries = 0;
+        fresh = false;
+        active = true;
+    };
     constexpr int kWaves = kBuildBlock / 64;
     __shared__ int s_wsum[kWaves];
-    __shared__ int s_base;
-    const int wave = threadIdx.x >> 6;
+    __shared__ int s_widle[kWaves];
+    __shared__ int s_base, s_qbase;
+    // lanes that want a body get consecutive queue entries (wave ranks, then wave offsets)
+    auto queue_entry = [&](bool want) {
+        const unsigned long long m = __ballot(want);
+        int off = s_qbase + __popcll(m & ((1ull << lane) - 1));
+        for (int q = 0; q < wave; ++q) off += s_widle[q];
+        return off;
+    };
+    auto body_of = [&](int j) { return exclude > 1 ? j + j / (exclude - 1) + 1 : j; };
+    bool drained = false;  // thread 0: the queue has no entries left
+    if (queued) {
+        if (lane == 0) s_widle[wave] = 64;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            s_qbase = atomicAdd(&S->bq, kBuildBlock);
+            drained = s_qbase >= nq;
+        }
+        __syncthreads();
+        const int j = queue_entry(true);
+        if (j < nq) start(body_of(j));
+    } else {
+        const int bi = w.begin + threadIdx.x;  // index in this phase's strided subset
+        const int bs = bi * stride;
+        if (bi < w.end && bs < n && (exclude == 0 || bs % exclude != 0)) start(bs);
+    }
     // block-synchronous rounds: one attempt per lane, then one allocation for the block
     while (__syncthreads_or(active)) {
         int k = 0, slot = 0, v = 0;
@@ -378,16 +415,27 @@ __global__ __launch_bounds__(kBuildBlock) void k_build(const SysDesc* __restrict
                 }
             }
         }
-        // one allocation for the whole block: wave prefix sums, then across the waves in LDS
+        // one allocation for the whole block: wave prefix sums, then across the waves in
+        // LDS; in the queued phase the same step claims the next bodies of the lanes that
+        // finish this round
+        const bool want = queued && (!active || k > 0);
         const int incl = wave_incl_scan(k);
         if (lane == 63) s_wsum[wave] = incl;
+        const int nwant = __popcll(__ballot(want));  // every lane takes part in the ballot
+        if (queued && lane == 0) s_widle[wave] = nwant;
         __syncthreads();
         if (threadIdx.x == 0) {
-            int t = 0;
+            int t = 0, u = 0;
             for (int q = 0; q < kWaves; ++q) t += s_wsum[q];
             s_base = t > 0 ? atomicAdd(&S->ncells, t) : 0;
+            if (queued) {
+                for (int q = 0; q < kWaves; ++q) u += s_widle[q];
+                s_qbase = u > 0 && !drained && !ld_c(&S->status) ? atomicAdd(&S->bq, u) : nq;
+                drained = drained || s_qbase >= nq;
+            }
         }
         __syncthreads();
+        const int qj = queued ? queue_entry(want) : nq;
         if (k > 0) {
             int woff = 0;
             for (int q = 0; q < wave; ++q) woff += s_wsum[q];
@@ -423,6 +471,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_build(const SysDesc* __restrict
             }
             active = false;
         }
+        if (want && qj < nq) start(body_of(qj));
     }
     // deepest level: block maximum, then an atomic only if it raises the stored value
     depth = wave_max_i(depth);
