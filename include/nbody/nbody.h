@@ -179,7 +179,10 @@ nb_status nb_stage_brute_force_range(nb_sim* sim, int64_t first, int64_t count);
  * 0.015; each body's velocity is reflected about the separation of every other body within
  * 0.03, in index order.  O(N^2); reads positions, writes velocities. */
 nb_status nb_stage_collision(nb_sim* sim);
-/* nsteps whole steps (BH: bounds..update; BF: brute force + update). */
+/* nsteps whole steps (BH: bounds..update; BF: brute force + update).  Without stage timing,
+ * brute-force simulations of up to 16384 bodies and no collision stage run the integrator
+ * inside the brute-force kernel (one launch per Euler step; results bit-identical to the
+ * separate stages). */
 nb_status nb_step(nb_sim* sim, int32_t nsteps);
 
 /* Tree inspection (synchronises).  Canonical forms: bodies in tree order with
