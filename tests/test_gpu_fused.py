@@ -3,7 +3,8 @@ run the integrator inside the brute-force kernel: Euler steps alternate between
 two position buffers (the kernel still reads the old positions as sources), and
 leapfrog's closing kick follows each acceleration.  The fused steps must equal
 the separate stage launches (brute force, then update) bit for bit, for every
-method, law and step count, at sizes on both sides of the 16384-body limit."""
+method, law and step count, at sizes on both sides of the 16384-body limit.
+Barnes-Hut steps replayed as graphs must equal timed, stage-by-stage steps."""
 import pytest
 
 from nbodysimulation_amd import _lib as L
@@ -92,3 +93,26 @@ def test_fused_alone_equals_batched(gpu):
         batched = ens.download(0)
     for a, b in zip(alone, batched):
         assert a.tobytes() == b.tobytes()
+
+
+@pytest.mark.parametrize("method", [L.EULER, L.LEAPFROG])
+@pytest.mark.parametrize("rules", [L.RULES_REFERENCE, L.RULES_STANDARD])
+def test_barnes_hut_graph_equals_timed_steps(gpu, method, rules):
+    """Untimed Barnes-Hut steps replay a captured graph; timed steps launch every stage
+    on its own between events.  The two must agree bit for bit, with the collision stage
+    on as well."""
+    n = 20000
+    state = ic.plummer(n, 80)
+    for collision in (False, True):
+        outs = []
+        for timed in (False, True):
+            cfg = make_config(n, L.BARNES_HUT, method=method, rules=rules, law=L.LAW_PLUMMER,
+                              time_stages=timed, collision=collision)
+            with NBodySim(cfg) as s:
+                s.upload(*state)
+                s.step(3)
+                s.step(4)
+                assert s.status() == L.NB_OK
+                outs.append(s.download())
+        for a, b in zip(*outs):
+            assert a.tobytes() == b.tobytes()
