@@ -316,7 +316,8 @@ nb_status nb_server_tick(nb_server* sv, int32_t* completed) {
         if (st) return st;
         ++sv->steps;
         std::vector<std::unique_ptr<Request>> keep;
-        for (auto& a : sv->active) {
+        for (size_t i = 0; i < sv->active.size(); ++i) {
+            auto& a = sv->active[i];
             const int bits = s[a->slot.slot].status;
             if (!bits) {
                 --a->steps_left;
@@ -325,7 +326,11 @@ nb_status nb_server_tick(nb_server* sv, int32_t* completed) {
             if (bits || a->steps_left <= 0) {
                 a->status = bits ? nb::status_from_bits(bits) : NB_OK;
                 st = download_request(sv, *a);
-                if (st) return st;
+                if (st) {  // keep this request and the unvisited ones in the active list
+                    for (size_t j = i; j < sv->active.size(); ++j) keep.push_back(std::move(sv->active[j]));
+                    sv->active = std::move(keep);
+                    return st;
+                }
                 unplace(sv, *a);
                 sv->finished_order.push_back(a->id);
                 sv->finished[a->id] = std::move(a);
