@@ -142,6 +142,15 @@ nb_status nb_sim_sync(nb_sim* sim);
 /* Status of device work queued so far (synchronises). */
 nb_status nb_sim_status(nb_sim* sim);
 nb_status nb_sim_clear_status(nb_sim* sim);
+/* Elastic recovery from NB_E_TREE_OVERFLOW (no counterpart: the reference resets its
+ * error word and the frame's tree is lost, build_tree.cl:109-114).  Re-creates the
+ * simulation's arena with a cell pool of `cells` (1 <= cells < 2^24) and the same
+ * bodies' positions, velocities, masses and accelerations; clear the status first,
+ * then repeat the failed step.  Under NB_EULER a failed step leaves the state at the
+ * last good step; under NB_LEAPFROG at the failed step's opening kick and drift.
+ * Barnes-Hut only (NB_E_STATE otherwise).  If the new pool cannot be allocated the
+ * old one is restored and the allocation error returned. */
+nb_status nb_sim_set_cell_capacity(nb_sim* sim, int32_t cells);
 
 /* Stages, in the order CMyApp::Update enqueues them (src/MyApp.cpp:583-598). */
 /* hybrid_reduce_root + parallel_reduce_root (kernels/boundary_reduce.cl:6-159);
@@ -179,7 +188,8 @@ nb_status nb_stage_brute_force_range(nb_sim* sim, int64_t first, int64_t count);
  * 0.015; each body's velocity is reflected about the separation of every other body within
  * 0.03, in index order.  O(N^2); reads positions, writes velocities. */
 nb_status nb_stage_collision(nb_sim* sim);
-/* nsteps whole steps (BH: bounds..update; BF: brute force + update).  Without stage timing,
+/* nsteps whole steps (BH: bounds..update; BF: brute force + update).  The step counter
+ * (nb_sim_step_count) advances only when every step succeeded.  Without stage timing,
  * brute-force simulations of up to 16384 bodies and no collision stage run the integrator
  * inside the brute-force kernel (one launch per Euler step; results bit-identical to the
  * separate stages). */

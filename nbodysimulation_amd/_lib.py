@@ -177,6 +177,7 @@ SIGNATURES = {
     "nb_stage_brute_force": (C.c_int, [_P]),
     "nb_stage_collision": (C.c_int, [_P]),
     "nb_stage_force_range": (C.c_int, [_P, C.c_int64, C.c_int64]),
+    "nb_sim_set_cell_capacity": (C.c_int, [_P, C.c_int32]),
     "nb_stage_brute_force_range": (C.c_int, [_P, C.c_int64, C.c_int64]),
     "nb_step": (C.c_int, [_P, C.c_int32]),
     "nb_tree_info_get": (C.c_int, [_P, C.POINTER(NbTreeInfo)]),

@@ -287,6 +287,47 @@ nb_status nb_sim_clear_status(nb_sim* sim) {
 
 static bool is_bh(const nb_sim* s) { return s->cfg.algorithm == NB_BARNES_HUT; }
 
+nb_status nb_sim_set_cell_capacity(nb_sim* sim, int32_t cells) {
+    if (!sim) return fail(NB_E_INVALID_ARG, "NULL sim");
+    if (!is_bh(sim)) return fail(NB_E_STATE, "the cell pool belongs to Barnes-Hut simulations");
+    if (cells < 1 || cells >= (1 << 24)) return fail(NB_E_INVALID_ARG, "cell capacity must be in [1, 2^24)");
+    Engine& e = sim->eng;
+    NB_HIPC(hipStreamSynchronize(e.stream));
+    // the bodies' state goes through the host; the new arena starts with an empty pool
+    const int n = sim->n;
+    std::vector<float4> x(std::max(n, 1)), v(std::max(n, 1)), a(std::max(n, 1));
+    if (n > 0) {
+        NB_HIPC(hipMemcpy(x.data(), e.node4 + e.hdesc[0].node_off, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        NB_HIPC(hipMemcpy(v.data(), e.vel4, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        NB_HIPC(hipMemcpy(a.data(), e.acc4, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    }
+    void* user_stream = e.own_stream ? nullptr : (void*)e.stream;
+    drop_graph(sim);
+    e.release();
+    nb_config c = sim->cfg;
+    c.cell_capacity = cells;
+    nb_status st = e.init({spec_of(&c, cells)}, user_stream);
+    if (st) {  // keep a usable simulation: the old pool size, the staged state
+        const std::string why = nb::last_error();
+        e.release();
+        if (e.init({spec_of(&sim->cfg, sim->cap)}, user_stream) != NB_OK) {
+            sim->have_state = false;
+            return fail(st, "cell pool resize failed (" + why + ") and the old pool could not be restored");
+        }
+    } else {
+        sim->cfg = c;
+        sim->cap = cells;
+    }
+    if (n > 0) {
+        NB_HIPC(hipMemcpy(e.node4 + e.hdesc[0].node_off, x.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+        NB_HIPC(hipMemcpy(e.vel4, v.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+        NB_HIPC(hipMemcpy(e.acc4, a.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    }
+    sim->have_box = sim->have_tree = sim->have_summary = sim->have_sorted = false;
+    sim->tree_clean = true;
+    return st;
+}
+
 static nb_status need(bool cond, const char* what) {
     return cond ? NB_OK : fail(NB_E_STATE, std::string("stage order: ") + what);
 }
@@ -596,8 +637,10 @@ nb_status nb_step(nb_sim* sim, int32_t nsteps) {
         nb_status st = read_stage_times(sim, nsteps);
         if (st) return st;
     }
-    after_steps(sim, nsteps);
-    return nb_sim_status(sim);
+    const nb_status st = nb_sim_status(sim);
+    // a failed step leaves the state at the last good step and the step counter unchanged
+    after_steps(sim, st == NB_OK ? nsteps : 0);
+    return st;
 }
 
 nb_status nb_step_async(nb_sim* sim, int32_t nsteps) {

@@ -124,8 +124,32 @@ class NBodySim:
         self.force()
         return box
 
-    def step(self, nsteps: int = 1):
-        self._call("nb_step", int(nsteps))
+    def step(self, nsteps: int = 1, grow: bool = False):
+        """nsteps whole steps.  grow=True (Barnes-Hut, Euler): a step that overflows the cell
+        pool is repeated after the pool has doubled (steps are then issued one at a time)."""
+        if not grow:
+            self._call("nb_step", int(nsteps))
+            return
+        if self.cfg.algorithm != L.BARNES_HUT or self.cfg.method != L.EULER:
+            raise ValueError("elastic cell pools need a Barnes-Hut simulation with the Euler method")
+        done = 0
+        while done < nsteps:
+            st = self.lib.nb_step(self.h, 1)
+            if st == L.NB_E_TREE_OVERFLOW and self.cell_capacity < (1 << 24) - 1:
+                self.clear_status()
+                self.set_cell_capacity(min(2 * self.cell_capacity, (1 << 24) - 1))
+                continue
+            L.check(st, "nb_step")
+            done += 1
+
+    @property
+    def cell_capacity(self) -> int:
+        return self.tree_info()["capacity"]
+
+    def set_cell_capacity(self, cells: int):
+        """Re-create the arena with a cell pool of `cells`, keeping the bodies' state."""
+        self._call("nb_sim_set_cell_capacity", int(cells))
+        self.cfg.cell_capacity = int(cells)
 
     def step_async(self, nsteps: int = 1):
         self._call("nb_step_async", int(nsteps))
