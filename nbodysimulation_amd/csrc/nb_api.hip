@@ -614,6 +614,7 @@ static void after_steps(nb_sim* sim, int nsteps) {
 }
 
 nb_status nb_step(nb_sim* sim, int32_t nsteps) {
+    nb::TraceRange tr("nb_step");
     if (!sim || nsteps < 0) return fail(NB_E_INVALID_ARG, "bad arguments");
     if (!sim->have_state) return fail(NB_E_STATE, "nb_step needs uploaded state");
     if (nsteps == 0 || sim->n == 0) return NB_OK;
@@ -644,6 +645,7 @@ nb_status nb_step(nb_sim* sim, int32_t nsteps) {
 }
 
 nb_status nb_step_async(nb_sim* sim, int32_t nsteps) {
+    nb::TraceRange tr("nb_step_async");
     if (!sim || nsteps < 0) return fail(NB_E_INVALID_ARG, "bad arguments");
     if (!sim->have_state) return fail(NB_E_STATE, "nb_step_async needs uploaded state");
     if (nsteps == 0 || sim->n == 0) return NB_OK;
@@ -1057,6 +1059,7 @@ nb_status nb_ens_download(nb_ensemble* e, int32_t sys, float* pos, float* vel, f
 }
 
 nb_status nb_ens_step(nb_ensemble* e, int32_t nsteps) {
+    nb::TraceRange tr("nb_ens_step");
     if (!e || nsteps < 0) return fail(NB_E_INVALID_ARG, "bad arguments");
     for (bool u : e->uploaded)
         if (!u) return fail(NB_E_STATE, "every system needs uploaded state");

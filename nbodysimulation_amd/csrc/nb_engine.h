@@ -14,6 +14,8 @@
 #include "nb_internal.h"
 #include "nbody/nbody.h"
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 namespace nb {
 
 // kernel launchers (nb_kernels.hip)
@@ -45,6 +47,16 @@ void launch_energy(hipStream_t s, int n, const float4* node4, const float4* vel4
                    double* partial, double* out);
 void launch_update(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
                    float4* node4, float4* vel4, const float4* acc4, int mode);
+
+// Host-side roctx range around a stage launch or an API call, visible to
+// rocprofv3 --marker-trace next to the kernels it launched (a no-op without a
+// profiler).  Under stream capture it marks the capture, not the replays.
+struct TraceRange {
+    explicit TraceRange(const char* name) { roctxRangePushA(name); }
+    ~TraceRange() { roctxRangePop(); }
+    TraceRange(const TraceRange&) = delete;
+    TraceRange& operator=(const TraceRange&) = delete;
+};
 
 void set_error(const std::string& msg);
 const char* last_error();

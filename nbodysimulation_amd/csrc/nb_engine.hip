@@ -307,29 +307,36 @@ nb_status Engine::init(const std::vector<SysSpec>& systems, void* user_stream) {
 }
 
 void Engine::bounds() {
+    TraceRange tr("nb:bounds");
     launch_bounds(stream, tab_len[T_BOUNDS], d_desc, d_state, tab[T_BOUNDS], node4, d_partial);
 }
 void Engine::tree_reset() {
+    TraceRange tr("nb:tree_reset");
     launch_tree_clear(stream, tab_len[T_CELLS], d_desc, d_state, tab[T_CELLS], child, cpend);
     launch_tree_init(stream, nactive, d_slots, d_state);
 }
 void Engine::build() {
+    TraceRange tr("nb:build");
     for (int ph = 0; ph < kBuildPhases; ++ph)
         launch_build(stream, tab_len[T_PHASE0 + ph], d_desc, d_state, tab[T_PHASE0 + ph], node4, child, cparent,
                      cpend, kPhaseStride[ph], ph == 0 ? 0 : kPhaseStride[ph - 1]);
 }
 void Engine::summarize() {
+    TraceRange tr("nb:summarize");
     launch_summarize(stream, tab_len[T_CELLS], d_desc, d_state, tab[T_CELLS], node4, child, cparent, cpend, ccount,
                      slotoff, cdata);
 }
 void Engine::sort() {
+    TraceRange tr("nb:sort");
     launch_sort(stream, tab_len[T_BODY], d_desc, d_state, tab[T_BODY], node4, child, slotoff, sorted);
 }
 void Engine::force() {
+    TraceRange tr("nb:force");
     launch_force_bh(stream, tab_len[T_GROUPS], kahan, d_desc, d_state, tab[T_GROUPS], node4, child, cdata, sorted,
                     acc4);
 }
 void Engine::force_range(int sys, int lo, int hi) {
+    TraceRange tr("nb:force_range");
     const int per = kForceVote * kForceWaves;
     const int start = lo / kForceVote * kForceVote;
     const int stop = (int)std::min<long long>(hdesc[sys].n, ((long long)hi + kForceVote - 1) / kForceVote * kForceVote);
@@ -338,6 +345,7 @@ void Engine::force_range(int sys, int lo, int hi) {
     launch_force_bh(stream, nblk, kahan, d_desc, d_state, tab[T_RANGE], node4, child, cdata, sorted, acc4);
 }
 void Engine::brute_range(int sys, int lo, int hi) {
+    TraceRange tr("nb:brute_force_range");
     const SysDesc& d = hdesc[sys];
     const int per = (kBfWaves >> d.bf_split_log2) * (64 >> d.bf_lanes_log2) * bf_tb;
     const int nblk = (hi - lo + per - 1) / per;
@@ -345,18 +353,22 @@ void Engine::brute_range(int sys, int lo, int hi) {
     launch_brute(stream, nblk, law, kahan, bf_tb, d_desc, d_state, tab[T_RANGE], node4, acc4, vel4, nullptr, -1);
 }
 void Engine::brute() {
+    TraceRange tr("nb:brute_force");
     launch_brute(stream, tab_len[T_BRUTE], law, kahan, bf_tb, d_desc, d_state, tab[T_BRUTE], node4, acc4, vel4,
                  nullptr, -1);
 }
 void Engine::brute_step(int mode, const float4* in, float4* out) {
+    TraceRange tr("nb:brute_force_step");
     launch_brute(stream, tab_len[T_BRUTE], law, kahan, bf_tb, d_desc, d_state, tab[T_BRUTE], in, acc4, vel4, out,
                  mode);
 }
 void Engine::update(int mode) {
+    TraceRange tr("nb:update");
     launch_update(stream, tab_len[T_UPDATE], d_desc, d_state, tab[T_UPDATE], node4, vel4, acc4, mode);
 }
 
 void Engine::collide() {
+    TraceRange tr("nb:collision");
     launch_collision(stream, tab_len[T_UPDATE], d_desc, d_state, tab[T_UPDATE], node4, vel4);
 }
 

@@ -264,6 +264,7 @@ nb_status nb_server_submit(nb_server* sv, const nb_system_spec* spec, int32_t st
 }
 
 nb_status nb_server_tick(nb_server* sv, int32_t* completed) {
+    nb::TraceRange tr("nb_server_tick");
     if (!sv) return sfail(NB_E_INVALID_ARG, "NULL server");
     if (completed) *completed = 0;
     int done = 0;
