@@ -222,6 +222,7 @@ def bench_bh(args):
         "ginteractions_executed_per_s": exec_inter * steps_per_s / 1e9,
         "interactions_per_step": {"algorithmic_per_body_traversal": i_alg, "executed_wave_vote": exec_inter},
         "stage_ms_per_step": {k: v / args.steps for k, v in stages.items()},
+        "stage_bandwidth": stage_bandwidth(n, tinfo["num_cells"], stages, args.steps),
         "stage_timing": "HIP events around every stage of the same steps launched stage by stage; value and "
                         "ms_per_step come from replayed graphs of the whole step",
         "host_round_trip": pcie,
@@ -235,6 +236,25 @@ def bench_bh(args):
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+# Algorithmic HBM bytes per step of the tree stages and the update (N bodies, C cells used):
+# bounds 12N, reset 32(C+1), build 16N+20C, summarize 16N+72C, sort 4N+40C, update 60N.
+STAGE_BYTES = {"bounds": lambda n, c: 12 * n, "reset": lambda n, c: 32 * (c + 1),
+               "build": lambda n, c: 16 * n + 20 * c, "summarize": lambda n, c: 16 * n + 72 * c,
+               "sort": lambda n, c: 4 * n + 40 * c, "update": lambda n, c: 60 * n}
+PEAK_HBM_GBS = 8000.0
+
+
+def stage_bandwidth(n, cells, stages, steps):
+    """Algorithmic bytes over each stage's event-timed duration, against MI355X's 8 TB/s."""
+    out = {}
+    for k, f in STAGE_BYTES.items():
+        ms = stages.get(k, 0.0) / steps
+        if ms > 0:
+            gbs = f(n, cells) / (ms * 1e-3) / 1e9
+            out[k] = {"algorithmic_bytes": f(n, cells), "ms": ms, "GB_s": gbs, "frac": gbs / PEAK_HBM_GBS}
+    return out
 
 
 def target_shares(pos, vel, mass, rules, theta, eps, G, dt, gpus=(2, 4, 8), reps=5):

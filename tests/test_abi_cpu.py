@@ -247,3 +247,17 @@ def test_autotune_host_helpers():
     assert abs(np.log(np.median(a)) - 0.5 * (np.log(4096) + np.log(2097152))) < 0.3
     assert autotune.choose_algorithm(8191, 8192) == L.BRUTE_FORCE_GLOBAL
     assert autotune.choose_algorithm(8192, 8192) == L.BARNES_HUT
+
+
+def test_bench_stage_bandwidth_formula():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    n, c, steps = 1 << 20, 500000, 10
+    stages = {"bounds": 0.2, "reset": 0.1, "build": 4.0, "summarize": 2.0, "sort": 1.0, "update": 0.2, "force": 20.0}
+    bw = bench.stage_bandwidth(n, c, stages, steps)
+    assert set(bw) == {"bounds", "reset", "build", "summarize", "sort", "update"}
+    assert bw["build"]["algorithmic_bytes"] == 16 * n + 20 * c
+    assert abs(bw["build"]["GB_s"] - (16 * n + 20 * c) / 0.4e-3 / 1e9) < 1e-9
+    assert abs(bw["update"]["frac"] - 60 * n / 0.02e-3 / 1e9 / 8000.0) < 1e-12
