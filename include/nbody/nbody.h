@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define NB_ABI_VERSION 1
+#define NB_ABI_VERSION 2
 
 typedef enum nb_status {
     NB_OK = 0,
@@ -262,6 +262,7 @@ typedef struct nb_system_spec {
     float theta;
     float epsilon;
     float dt;
+    int32_t collision;      /* 1: the collision stage runs after this system's update */
 } nb_system_spec;
 
 typedef struct nb_ensemble nb_ensemble;

@@ -11,6 +11,7 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libnbody.so")
+ABI_VERSION = 2  # NB_ABI_VERSION of the structs mirrored below
 
 NB_OK = 0
 NB_E_INVALID_ARG = 1
@@ -80,6 +81,7 @@ class NbSystemSpec(C.Structure):
         ("theta", C.c_float),
         ("epsilon", C.c_float),
         ("dt", C.c_float),
+        ("collision", C.c_int32),
     ]
 
 
@@ -234,6 +236,8 @@ def load():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    if lib.nb_abi_version() != ABI_VERSION:  # struct layouts of this module and the library differ
+        raise RuntimeError(f"{LIB_PATH} has ABI version {lib.nb_abi_version()}, this module {ABI_VERSION}: rebuild it")
     _lib = lib
     return lib
 

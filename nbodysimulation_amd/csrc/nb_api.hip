@@ -155,6 +155,7 @@ static SysSpec spec_of(const nb_config* c, int cap) {
     s.theta = c->theta;
     s.eps = c->epsilon;
     s.dt = c->dt;
+    s.collision = c->collision;
     return s;
 }
 
@@ -435,7 +436,7 @@ nb_status nb_stage_collision(nb_sim* sim) {
     if (!sim) return fail(NB_E_INVALID_ARG, "NULL sim");
     nb_status st = need(sim->have_state, "collision needs uploaded state");
     if (st) return st;
-    sim->eng.collide();
+    sim->eng.collide(true);  // on request, whatever the configuration says
     return NB_OK;
 }
 
@@ -967,6 +968,7 @@ static void ens_enqueue_step(nb_ensemble* e) {
     if (lf) e->eng.update(2);
     ens_enqueue_force(e);
     e->eng.update(lf ? 1 : 0);
+    if (e->eng.any_collision) e->eng.collide();  // the systems that ask for it, after the integrator
 }
 
 extern "C" {
@@ -1133,6 +1135,7 @@ nb_status spec_from_system(const nb_system_spec& s, int method, int law, int acc
     c.epsilon = s.epsilon;
     c.dt = s.dt;
     c.cell_capacity = s.cell_capacity;
+    c.collision = s.collision;
     nb_status st = validate(&c);
     if (st) return st;
     const int cap = c.algorithm == NB_BARNES_HUT ? (c.cell_capacity > 0 ? c.cell_capacity : 2 * c.num_bodies + 1024) : 0;

@@ -40,7 +40,8 @@ void launch_brute(hipStream_t s, int nblk, int law, bool kahan, int tb, const Sy
                   const BlkWork* work, const float4* node4, float4* acc4, float4* vel4, float4* out4, int mode);
 void launch_range_table(hipStream_t s, BlkWork* out, int nblk, int sys, int start, int stop, int lo, int hi,
                         int per);
-void launch_collision(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
+// all = false: only the systems whose SysDesc asks for the collision stage
+void launch_collision(hipStream_t s, int nblk, bool all, const SysDesc* sd, const SysState* st, const BlkWork* work,
                       const float4* node4, float4* vel4);
 void launch_vertices(hipStream_t s, int n, const float4* node4, float4* out);
 void launch_energy(hipStream_t s, int n, const float4* node4, const float4* vel4, int law, double eps, double G,
@@ -70,6 +71,7 @@ struct SysSpec {
     int cap = 0;  // cells (Barnes-Hut only)
     int rules = 0, algo = 0, law = 1, accum = 0, method = 2;
     float G = 1.f, theta = 0.5f, eps = 0.01f, dt = 1e-3f;
+    int collision = 0;  // the collision stage runs after this system's update
 };
 
 // A system placed in the arena.
@@ -158,7 +160,9 @@ struct Engine {
     void force_range(int sys, int lo, int hi);
     void brute_range(int sys, int lo, int hi);
     void update(int mode);
-    void collide();
+    // collision stage of the systems that ask for it (all: of every system)
+    void collide(bool all = false);
+    bool any_collision = false;
 
     nb_status download_state(std::vector<SysState>& out);
 };

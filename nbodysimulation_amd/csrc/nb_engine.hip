@@ -184,7 +184,7 @@ nb_status Engine::activate(const std::vector<Slot>& slots) {
     std::vector<int> ids;
     std::vector<char> seen(capc.systems, 0);
     long long bf_waves2 = 0;  // brute-force waves of the batch at two targets per lane
-    any_bh = any_bf = false;
+    any_bh = any_bf = any_collision = false;
     for (const Slot& sl : slots) {
         if (sl.slot < 0 || sl.slot >= capc.systems || seen[sl.slot]) {
             set_error("slot index out of range or used twice");
@@ -219,6 +219,8 @@ nb_status Engine::activate(const std::vector<Slot>& slots) {
         d.eps = p.eps;
         d.dt = p.dt;
         d.ueps = p.algo == NB_BRUTE_FORCE_LOCAL ? 0.1f : 0.01f;
+        d.collision = p.collision ? 1 : 0;
+        any_collision = any_collision || (p.collision && d.n > 0);
         const int s = sl.slot;
         ids.push_back(s);
         if (bh && d.n > 0) {
@@ -367,9 +369,9 @@ void Engine::update(int mode) {
     launch_update(stream, tab_len[T_UPDATE], d_desc, d_state, tab[T_UPDATE], node4, vel4, acc4, mode);
 }
 
-void Engine::collide() {
+void Engine::collide(bool all) {
     TraceRange tr("nb:collision");
-    launch_collision(stream, tab_len[T_UPDATE], d_desc, d_state, tab[T_UPDATE], node4, vel4);
+    launch_collision(stream, tab_len[T_UPDATE], all, d_desc, d_state, tab[T_UPDATE], node4, vel4);
 }
 
 nb_status Engine::download_state(std::vector<SysState>& out) {

@@ -57,7 +57,7 @@ struct SysDesc {
     int bbox_blocks;  // number of bounds blocks of this system
     int bf_split_log2;  // brute force: log2 of the waves sharing one target group
     int bf_lanes_log2;  // brute force: log2 of the lanes sharing one target
-    int pad;
+    int collision;      // 1: the batch's collision stage includes this system
 };
 
 struct SysState {

@@ -941,11 +941,12 @@ constexpr float kCollisionRadius = 0.015f;
 
 __global__ __launch_bounds__(256) void k_collision(const SysDesc* __restrict__ sd, const SysState* __restrict__ st,
                                                    const BlkWork* __restrict__ work,
-                                                   const float4* __restrict__ node4, float4* __restrict__ vel4) {
+                                                   const float4* __restrict__ node4, float4* __restrict__ vel4,
+                                                   int all) {
     __shared__ float4 tile[4][64];
     const BlkWork w = work[blockIdx.x];
     const SysDesc d = sd[w.sys];
-    if (st[w.sys].status) return;
+    if (st[w.sys].status || !(all || d.collision)) return;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int i = min(w.begin + (int)threadIdx.x, w.end - 1);
     const float4* __restrict__ nodes = node4 + d.node_off;
@@ -1127,9 +1128,9 @@ void launch_brute(hipStream_t s, int nblk, int law, bool kahan, int tb, const Sy
     NB_BF(0, false, 2); NB_BF(0, true, 2); NB_BF(1, false, 2); NB_BF(1, true, 2);
 #undef NB_BF
 }
-void launch_collision(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
+void launch_collision(hipStream_t s, int nblk, bool all, const SysDesc* sd, const SysState* st, const BlkWork* work,
                       const float4* node4, float4* vel4) {
-    NB_LAUNCH(k_collision, nblk, 256, s, sd, st, work, node4, vel4);
+    NB_LAUNCH(k_collision, nblk, 256, s, sd, st, work, node4, vel4, all ? 1 : 0);
 }
 void launch_vertices(hipStream_t s, int n, const float4* node4, float4* out) {
     NB_LAUNCH(k_vertices, (n + 255) / 256, 256, s, n, node4, out);

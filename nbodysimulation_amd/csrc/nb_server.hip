@@ -311,6 +311,7 @@ nb_status nb_server_tick(nb_server* sv, int32_t* completed) {
         if (lf) g.update(2);
         enqueue_force(g);
         g.update(lf ? 1 : 0);
+        if (g.any_collision) g.collide();  // the requests that ask for it, after the integrator
         NB_HIPS(hipGetLastError());
         std::vector<nb::SysState> s;
         nb_status st = g.download_state(s);

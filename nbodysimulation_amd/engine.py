@@ -240,9 +240,11 @@ class NBodySim:
 
 
 def system_spec(n: int, algorithm: int = L.BARNES_HUT, *, rules: int = L.RULES_STANDARD, G: float = 1.0,
-                theta: float = 0.5, eps: float = 0.01, dt: float = 1e-3, cell_capacity: int = 0) -> L.NbSystemSpec:
+                theta: float = 0.5, eps: float = 0.01, dt: float = 1e-3, cell_capacity: int = 0,
+                collision: bool = False) -> L.NbSystemSpec:
     sp = L.NbSystemSpec()
     sp.num_bodies, sp.algorithm, sp.rules, sp.cell_capacity = n, algorithm, rules, cell_capacity
+    sp.collision = int(collision)
     sp.gravitational_constant, sp.theta, sp.epsilon, sp.dt = G, theta, eps, dt
     return sp
 

@@ -40,7 +40,7 @@ def test_config_defaults_match_reference(nbl):
     assert c.max_octree_depth == 32
     assert np.float32(c.theta) == np.float32(1.5) and np.float32(c.epsilon) == np.float32(0.01)
     assert np.float32(c.gravitational_constant) == np.float32(6.67e-11)
-    assert nbl.nb_abi_version() == 1
+    assert nbl.nb_abi_version() == 2
 
 
 @pytest.mark.parametrize("n,cus,want", [(20000, 256, 262144), (1048576, 256, 2097152), (7, 1, 1024),

@@ -104,3 +104,31 @@ def test_step_async_then_status(gpu):
             t.step(4)
             for a, b in zip(s.download(), t.download()):
                 assert a.tobytes() == b.tobytes()
+
+
+def test_collision_per_system_in_a_batch(gpu):
+    """The collision stage runs for the systems whose spec asks for it, after the batch's
+    integrator; each system equals the same system stepped alone with the same setting."""
+    specs = [system_spec(3000, collision=True), system_spec(2000, L.BRUTE_FORCE_GLOBAL, collision=True),
+             system_spec(2500), system_spec(1500, L.BRUTE_FORCE_GLOBAL)]
+    states = [ic.uniform_ball(sp.num_bodies, 70 + i, radius=0.3) for i, sp in enumerate(specs)]
+    with NBodyEnsemble(specs, law=L.LAW_PLUMMER) as ens:
+        for i, st in enumerate(states):
+            ens.upload(i, *st)
+        ens.step(4)
+        mixed = [ens.download(i) for i in range(len(specs))]
+    for i, (sp, st) in enumerate(zip(specs, states)):
+        cfg = make_config(sp.num_bodies, sp.algorithm, rules=sp.rules, law=L.LAW_PLUMMER,
+                          cell_capacity=2 * sp.num_bodies + 1024, collision=bool(sp.collision))
+        with NBodySim(cfg) as s:
+            s.upload(*st)
+            s.step(4)
+            alone = s.download()
+        for a, b in zip(alone, mixed[i]):
+            assert a.tobytes() == b.tobytes(), f"system {i}"
+    # the setting matters: a colliding ball differs from the same ball without the stage
+    with NBodySim(make_config(3000, L.BARNES_HUT, rules=L.RULES_STANDARD, law=L.LAW_PLUMMER,
+                              cell_capacity=2 * 3000 + 1024)) as s:
+        s.upload(*states[0])
+        s.step(4)
+        assert s.download()[1].tobytes() != mixed[0][1].tobytes()

@@ -119,3 +119,26 @@ def test_empty_system_request(gpu):
     alone = run_alone(system_spec(500), 5, (p, v, m), L.EULER)
     for x, y in zip(alone, res):
         assert x.tobytes() == y.tobytes()
+
+
+def test_collision_request_in_the_mix(gpu):
+    """A request with the collision stage joins a batch of requests without it; each equals
+    its run alone."""
+    ball = ic.uniform_ball(2000, 8, radius=0.3)
+    other = ic.plummer(3000, 9)
+    with NBodyServer(16 * MiB, 4) as sv:
+        a = sv.submit(system_spec(2000, collision=True), 5, *ball)
+        b = sv.submit(system_spec(3000), 3, *other)
+        status = drain(sv)
+        assert status == {a: L.NB_OK, b: L.NB_OK}
+        ra, rb = sv.fetch(a), sv.fetch(b)
+    cfg = make_config(2000, L.BARNES_HUT, rules=L.RULES_STANDARD, law=L.LAW_PLUMMER, cell_capacity=2 * 2000 + 1024,
+                      collision=True)
+    with NBodySim(cfg) as s:
+        s.upload(*ball)
+        s.step(5)
+        alone = s.download()
+    for x, y in zip(alone, ra):
+        assert x.tobytes() == y.tobytes()
+    for x, y in zip(run_alone(system_spec(3000), 3, other, L.EULER), rb):
+        assert x.tobytes() == y.tobytes()
