@@ -30,7 +30,8 @@ void launch_summarize(hipStream_t s, int nblk, const SysDesc* sd, const SysState
                       float4* node4, const int* child, const int* cparent, int* cpend, int* ccount, int* slotoff,
                       float4* cdata);
 void launch_sort(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work, const float4* node4,
-                 const int* child, const int* slotoff, int* sorted);
+                 const int* child, const int* slotoff, int* sorted, const float4* acc4, int* keys);
+void launch_force_order(hipStream_t s, const BlkWork* in, int len, int* keys, BlkWork* out);
 void launch_force_bh(hipStream_t s, int nblk, bool kahan, const SysDesc* sd, const SysState* st,
                      const BlkWork* work, const float4* node4, const int* child, const float4* cdata,
                      const int* sorted, float4* acc4);
@@ -120,12 +121,15 @@ struct Engine {
     SysState* d_state = nullptr;
     float* d_partial = nullptr;
     int* d_slots = nullptr;  // slots of the active systems (tree_init touches only these)
+    int* gkey = nullptr;     // per force workgroup: dispatch-order key (k_sort -> k_force_order)
+    bool lpt = false;        // force workgroups dispatched longest first (few waves per SIMD)
     int nactive = 0;
 
     // block tables (device) with their capacities and current lengths
-    // T_RANGE is filled on the device by each target-range force launch
+    // T_RANGE (target-range force launches) and T_FORDER (T_GROUPS in dispatch
+    // order) are filled on the device
     enum Table { T_BOUNDS, T_BODY, T_CELLS, T_GROUPS, T_BRUTE, T_UPDATE, T_PHASE0, T_PHASE1, T_PHASE2, T_PHASE3,
-                 T_RANGE, T_COUNT };
+                 T_RANGE, T_FORDER, T_COUNT };
     BlkWork* tab[T_COUNT] = {};
     int tab_cap[T_COUNT] = {};
     int tab_len[T_COUNT] = {};

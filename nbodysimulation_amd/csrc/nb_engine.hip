@@ -131,6 +131,7 @@ nb_status Engine::reserve(const Capacity& c, int law_, bool kahan_, void* user_s
     tab_cap[T_BRUTE] = blocks_bound(c.bodies, 16, S);
     tab_cap[T_UPDATE] = blocks_bound(c.bodies, 256, S);
     tab_cap[T_RANGE] = blocks_bound(c.bodies, 16, S);  // the smallest share per block of either force kernel
+    tab_cap[T_FORDER] = tab_cap[T_GROUPS];
     for (int ph = 0; ph < kBuildPhases; ++ph)
         tab_cap[T_PHASE0 + ph] = blocks_bound((c.bodies + kPhaseStride[ph] - 1) / kPhaseStride[ph], kBuildBlock, S);
 
@@ -152,6 +153,7 @@ nb_status Engine::reserve(const Capacity& c, int law_, bool kahan_, void* user_s
         {(void**)&d_state, sizeof(SysState) * S},
         {(void**)&d_partial, sizeof(float) * 7 * (size_t)tab_cap[T_BOUNDS]},
         {(void**)&d_slots, sizeof(int) * S},
+        {(void**)&gkey, sizeof(int) * (size_t)std::max(tab_cap[T_GROUPS], 1)},
     };
     for (int t = 0; t < T_COUNT; ++t) regs.push_back({(void**)&tab[t], sizeof(BlkWork) * (size_t)tab_cap[t]});
     size_t total = 0;
@@ -173,6 +175,7 @@ nb_status Engine::reserve(const Capacity& c, int law_, bool kahan_, void* user_s
     NB_HIP(hipMemsetAsync(ccount, 0, sizeof(int) * nc, stream));
     NB_HIP(hipMemsetAsync(slotoff, 0, sizeof(int) * 8 * nc, stream));
     NB_HIP(hipMemsetAsync(d_state, 0, sizeof(SysState) * S, stream));
+    NB_HIP(hipMemsetAsync(gkey, 0, sizeof(int) * (size_t)std::max(tab_cap[T_GROUPS], 1), stream));
     launch_tree_init(stream, S, nullptr, d_state);
     NB_HIP(hipGetLastError());
     NB_HIP(hipStreamSynchronize(stream));
@@ -230,6 +233,7 @@ nb_status Engine::activate(const std::vector<Slot>& slots) {
             d.bbox_blocks = (int)t[T_BOUNDS].size() - d.bbox_first;
             add_ranges(t[T_BODY], s, d.n, 256);
             add_ranges(t[T_CELLS], s, d.cap, 256);
+            d.grp_first = (int)t[T_GROUPS].size();
             add_ranges(t[T_GROUPS], s, d.n, kForceVote * kForceWaves);
             for (int ph = 0; ph < kBuildPhases; ++ph)
                 add_ranges(t[T_PHASE0 + ph], s, (d.n + kPhaseStride[ph] - 1) / kPhaseStride[ph], kBuildBlock);
@@ -259,6 +263,9 @@ nb_status Engine::activate(const std::vector<Slot>& slots) {
         if (!t[k].empty())
             NB_HIP(hipMemcpyAsync(tab[k], t[k].data(), sizeof(BlkWork) * t[k].size(), hipMemcpyHostToDevice, stream));
     }
+    // longest-first dispatch only where a launch holds at most two waves per SIMD slot
+    // (8 waves x 4 SIMDs per CU): with more, the last workgroups matter less than locality
+    lpt = tab_len[T_GROUPS] <= kOrderMax && (long long)tab_len[T_GROUPS] * kForceWaves <= 2LL * 32 * cus;
     NB_HIP(hipMemcpyAsync(d_desc, hdesc.data(), sizeof(SysDesc) * capc.systems, hipMemcpyHostToDevice, stream));
     if (!ids.empty()) NB_HIP(hipMemcpyAsync(d_slots, ids.data(), sizeof(int) * ids.size(), hipMemcpyHostToDevice, stream));
     NB_HIP(hipStreamSynchronize(stream));
@@ -330,12 +337,14 @@ void Engine::summarize() {
 }
 void Engine::sort() {
     TraceRange tr("nb:sort");
-    launch_sort(stream, tab_len[T_BODY], d_desc, d_state, tab[T_BODY], node4, child, slotoff, sorted);
+    launch_sort(stream, tab_len[T_BODY], d_desc, d_state, tab[T_BODY], node4, child, slotoff, sorted, acc4,
+                lpt ? gkey : nullptr);
 }
 void Engine::force() {
     TraceRange tr("nb:force");
-    launch_force_bh(stream, tab_len[T_GROUPS], kahan, d_desc, d_state, tab[T_GROUPS], node4, child, cdata, sorted,
-                    acc4);
+    if (lpt) launch_force_order(stream, tab[T_GROUPS], tab_len[T_GROUPS], gkey, tab[T_FORDER]);
+    launch_force_bh(stream, tab_len[T_GROUPS], kahan, d_desc, d_state, tab[lpt ? T_FORDER : T_GROUPS], node4, child,
+                    cdata, sorted, acc4);
 }
 void Engine::force_range(int sys, int lo, int hi) {
     TraceRange tr("nb:force_range");

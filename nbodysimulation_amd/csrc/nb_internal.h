@@ -30,6 +30,7 @@ constexpr int kBfTileStride = 4 * 17;  // a wave's LDS tile: up to 4 parts of 16
 #endif
 constexpr int kForceWaves = NB_FORCE_WAVES;  // Barnes-Hut force workgroup: waves of 64
 constexpr int kForceVote = 64;
+constexpr int kOrderMax = 32768;  // force workgroups that k_force_order can order (keys in LDS)
 constexpr int kBfFuseMax = 16384;  // single systems up to this size run the integrator inside k_brute  // bodies per opening decision: one wave, one lane per body
 constexpr int kEnergyLen = 9;   // kinetic, potential, p[3], L[3], mass
 constexpr int kEmpty = -1;
@@ -58,6 +59,8 @@ struct SysDesc {
     int bf_split_log2;  // brute force: log2 of the waves sharing one target group
     int bf_lanes_log2;  // brute force: log2 of the lanes sharing one target
     int collision;      // 1: the batch's collision stage includes this system
+    int grp_first;      // first workgroup of this system in the force block table
+    int pad[3];
 };
 
 struct SysState {

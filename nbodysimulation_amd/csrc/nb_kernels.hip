@@ -13,6 +13,7 @@
 //   k_collision   update_collision_local                     kernels/collision_update.cl:3-83
 //   k_vertices    copy_vertices                              kernels/copy_vertices.cl:3-17
 //   k_range_table block table of one target range (no counterpart: one GPU's share of a force pass)
+//   k_force_order longest-first dispatch order of the force workgroups (no counterpart)
 //   k_energy_f64, k_energy_final   FP64 conservation diagnostics (no counterpart)
 //
 // Memory-model notes (gfx950 has one L2 per XCD):
@@ -573,10 +574,13 @@ __global__ __launch_bounds__(256) void k_summarize(const SysDesc* __restrict__ s
 // re-derives its path (the same octant digits as the build) and adds them up:
 // no waiting, no dependence on the order in which cells were allocated.
 
+// keys (optional): per force workgroup of the coming force pass, the largest
+// interaction count that its waves' first bodies had in the previous force pass
+// (acc4.w), the dispatch-order key of k_force_order.
 __global__ __launch_bounds__(256) void k_sort(const SysDesc* __restrict__ sd, SysState* st,
                                               const BlkWork* __restrict__ work, const float4* __restrict__ node4,
                                               const int* __restrict__ child, const int* __restrict__ slotoff,
-                                              int* sorted) {
+                                              int* sorted, const float4* __restrict__ acc4, int* keys) {
     const BlkWork w = work[blockIdx.x];
     const SysDesc d = sd[w.sys];
     SysState* S = st + w.sys;
@@ -595,7 +599,12 @@ __global__ __launch_bounds__(256) void k_sort(const SysDesc* __restrict__ sd, Sy
         const int v = C[8 * node + dg];
         rank += O[8 * node + dg];
         if (v == b) {
-            if (rank < n) sorted[d.body_off + rank] = b;
+            if (rank < n) {
+                sorted[d.body_off + rank] = b;
+                if (keys && (rank & (kForceVote - 1)) == 0)
+                    atomicMax(keys + d.grp_first + rank / (kForceVote * kForceWaves),
+                              (int)fminf(fmaxf(acc4[d.body_off + b].w, 0.0f), 1e6f));
+            }
             return;
         }
         if (v < n) break;  // not this body's slot: inconsistent tree
@@ -695,6 +704,51 @@ __global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __
         if (KAHAN) { ax -= kx; ay -= ky; az -= kz; }
         acc4[d.body_off + b] = make_float4(ax * d.G, ay * d.G, az * d.G, (float)nint);
     }
+}
+
+// Dispatch order of the force workgroups, longest first (LPT list scheduling),
+// for launches of at most two waves per SIMD slot, where the last workgroups to
+// start decide when the kernel ends.  keys[i] (from k_sort; see there) estimates
+// workgroup i's time; workgroups are binned by it (64 bins up to the largest) and
+// written out largest bin first, and the keys are cleared for the next pass.  The
+// order decides only when a workgroup starts, never what it computes.  One
+// workgroup of 1024 threads; the keys stay in LDS.
+constexpr int kOrderBins = 64;
+
+__global__ __launch_bounds__(1024) void k_force_order(const BlkWork* __restrict__ in, int len, int* keys,
+                                                      BlkWork* __restrict__ out) {
+    __shared__ int key[kOrderMax];
+    __shared__ int hist[kOrderBins];
+    __shared__ int wmax[16];
+    int m = 0;
+    for (int i = threadIdx.x; i < len; i += blockDim.x) {
+        const int k = min(max(keys[i], 0), 1 << 24);
+        keys[i] = 0;
+        key[i] = k;
+        m = max(m, k);
+    }
+    m = wave_max_i(m);
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    if (threadIdx.x < kOrderBins) hist[threadIdx.x] = 0;
+    __syncthreads();
+    int top = 0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) top = max(top, wmax[q]);
+    const long long scale = (long long)top + 1;
+    for (int i = threadIdx.x; i < len; i += blockDim.x) {
+        key[i] = kOrderBins - 1 - (int)((long long)key[i] * kOrderBins / scale);
+        atomicAdd(&hist[key[i]], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int sum = 0;
+        for (int b = 0; b < kOrderBins; ++b) {
+            const int c = hist[b];
+            hist[b] = sum;
+            sum += c;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < len; i += blockDim.x) out[atomicAdd(&hist[key[i]], 1)] = in[i];
 }
 
 // Block table of one target range [lo, hi) of system `sys`: blocks of `per` items
@@ -1103,8 +1157,11 @@ void launch_summarize(hipStream_t s, int nblk, const SysDesc* sd, const SysState
     NB_LAUNCH(k_summarize, nblk, 256, s, sd, st, work, node4, child, cparent, cpend, ccount, slotoff, cdata);
 }
 void launch_sort(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work, const float4* node4,
-                 const int* child, const int* slotoff, int* sorted) {
-    NB_LAUNCH(k_sort, nblk, 256, s, sd, st, work, node4, child, slotoff, sorted);
+                 const int* child, const int* slotoff, int* sorted, const float4* acc4, int* keys) {
+    NB_LAUNCH(k_sort, nblk, 256, s, sd, st, work, node4, child, slotoff, sorted, acc4, keys);
+}
+void launch_force_order(hipStream_t s, const BlkWork* in, int len, int* keys, BlkWork* out) {
+    if (len > 0 && len <= kOrderMax) NB_LAUNCH(k_force_order, 1, 1024, s, in, len, keys, out);
 }
 void launch_force_bh(hipStream_t s, int nblk, bool kahan, const SysDesc* sd, const SysState* st,
                      const BlkWork* work, const float4* node4, const int* child, const float4* cdata,
