@@ -261,7 +261,9 @@ def target_shares(pos, vel, mass, rules, theta, eps, G, dt, gpus=(2, 4, 8), reps
     """One GPU's share of the force pass when the targets are split over G GPUs that each hold
     the whole tree: the force stage over [r*N/G, (r+1)*N/G) of the tree order, timed with HIP
     events on the simulation's stream (mean of `reps` launches after one warm-up), for every rank
-    r.  The all-gather of positions that such a split needs is not part of it."""
+    r.  The all-gather of positions that such a split needs is not part of it.  The full pass is timed
+    the same way; repeated without a new sort it has no dispatch-order keys, so like the shares it
+    runs its workgroups in table order."""
     import ctypes as C
 
     from nbodysimulation_amd import _lib as L
