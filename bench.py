@@ -180,7 +180,7 @@ def bench_bh(args):
     groups = np.sort(np.random.default_rng(7).choice(n, sample, replace=False)).astype(np.int32)
     _, _, ni = tree.force(theta, eps, G, vote=1, groups=groups)
     t_d = time.perf_counter()
-    members = tree.sorted[groups]
+    members = tree.force_order[groups]
     i_alg = float(ni[members].mean()) * n
     acc_dummy = np.zeros((n, 3), np.float32)
     t_e = time.perf_counter()

@@ -177,10 +177,10 @@ nb_status nb_stage_brute_force(nb_sim* sim);
  * update_local, kernels/brute_force_update.cl:108-228) run over a sub-range of its
  * NDRange; the reference itself never launches one.  Accelerations of the targets
  * [first, first+count) are written, every other body's are left as they were.
- * nb_stage_force_range counts targets in tree order (nb_export_bodies' `sorted`); the
- * vote groups stay those of the full pass (64 consecutive bodies from tree position 0),
- * so every written acceleration equals the full pass's bit for bit, wherever the range
- * starts or ends.  nb_stage_brute_force_range counts targets in body index order; its
+ * nb_stage_force_range counts targets in force order (nb_export_force_order); the
+ * vote groups stay those of the full pass (64 consecutive bodies from position 0 of that
+ * order), so every written acceleration equals the full pass's bit for bit, wherever the
+ * range starts or ends.  nb_stage_brute_force_range counts targets in body index order; its
  * results also equal the full pass's.  A range outside [0, N] is NB_E_INVALID_ARG. */
 nb_status nb_stage_force_range(nb_sim* sim, int64_t first, int64_t count);
 nb_status nb_stage_brute_force_range(nb_sim* sim, int64_t first, int64_t count);
@@ -201,6 +201,14 @@ nb_status nb_step(nb_sim* sim, int32_t nsteps);
 nb_status nb_tree_info_get(nb_sim* sim, nb_tree_info* info);
 nb_status nb_export_bodies(nb_sim* sim, int32_t* sorted, uint8_t* paths, int32_t* path_len);
 nb_status nb_export_cells(nb_sim* sim, int32_t* level, int32_t* slot, int32_t* count, float* com4);
+/* The force order (N body indices).  STANDARD rules: depth-first like `sorted`, but each
+ * cell's children taken along a 3-D Hilbert curve (Skilling's index on the octants), so
+ * 64 consecutive bodies span less space than in `sorted`.  REFERENCE rules: `sorted`
+ * itself (those cells are not boxes in y).  No counterpart in the reference, whose vote
+ * groups are the bodies of a work-group in index order (calculate_forces.cl:84-148).  The
+ * Barnes-Hut force pass forms its wave-wide opening votes from 64 consecutive bodies of
+ * this order; nb_stage_force_range counts in it. */
+nb_status nb_export_force_order(nb_sim* sim, int32_t* order);
 /* Opening thresholds per level (NB_MAX_LEVEL+1 floats) of the current box. */
 nb_status nb_export_dq(nb_sim* sim, float* dq);
 /* Interactions the last nb_stage_force performed, summed over bodies (synchronises). */

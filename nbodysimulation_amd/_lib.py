@@ -185,6 +185,7 @@ SIGNATURES = {
     "nb_tree_info_get": (C.c_int, [_P, C.POINTER(NbTreeInfo)]),
     "nb_export_bodies": (C.c_int, [_P, _IP, _U8P, _IP]),
     "nb_export_cells": (C.c_int, [_P, _IP, _IP, _IP, _FP]),
+    "nb_export_force_order": (C.c_int, [_P, _IP]),
     "nb_export_dq": (C.c_int, [_P, _FP]),
     "nb_force_interactions": (C.c_int, [_P, _I64P]),
     "nb_stage_times": (C.c_int, [_P, _FP]),

@@ -896,6 +896,17 @@ nb_status nb_export_bodies(nb_sim* sim, int32_t* sorted, uint8_t* paths, int32_t
     return NB_OK;
 }
 
+nb_status nb_export_force_order(nb_sim* sim, int32_t* order) {
+    if (!sim || !order) return fail(NB_E_INVALID_ARG, "NULL argument");
+    if (!sim->have_sorted) return fail(NB_E_STATE, "no force order to export");
+    Engine& e = sim->eng;
+    if (sim->n > 0) {
+        NB_HIPC(hipMemcpyAsync(order, e.forder, sizeof(int) * sim->n, hipMemcpyDeviceToHost, e.stream));
+        NB_HIPC(hipStreamSynchronize(e.stream));
+    }
+    return NB_OK;
+}
+
 nb_status nb_export_cells(nb_sim* sim, int32_t* level, int32_t* slot, int32_t* count, float* com4) {
     if (!sim) return fail(NB_E_INVALID_ARG, "NULL sim");
     HostTree t;

@@ -10,11 +10,13 @@
 //                                    cells  [node_off+n, node_off+n+cap) as (com, mass)
 //   vel4   float4[sum n_s]           velocity (w unused)
 //   acc4   float4[sum n_s]           acceleration (w = interactions of the last force pass)
-//   sorted int[sum n_s]              local body ids in tree order
+//   sorted int[sum n_s]              local body ids in tree order (children in slot order)
+//   forder int[sum n_s]              local body ids in force order (children along a Hilbert
+//                                    curve); the force pass's vote groups are its 64-runs
 //   child  int[8 * sum cap_s]        per cell 8 slots: -1 empty, -2 locked,
 //                                    [0,n) body, n+ci cell ci (local numbering)
 //   cparent, cpend, ccount           int[sum cap_s]   parent, unfinished child cells, bodies below
-//   slotoff                          int[8 * sum cap_s] bodies in earlier slots of the same cell
+//   slotoff                          int[8 * sum cap_s] bodies in slots 0..k of the same cell
 //   cdata                            float4[8 * sum cap_s] children (x,y,z,m) in slot order
 #pragma once
 #include <stdint.h>

@@ -524,29 +524,34 @@ __global__ __launch_bounds__(256) void k_summarize(const SysDesc* __restrict__ s
         int vs[8] = {a.x, a.y, a.z, a.w, bq.x, bq.y, bq.z, bq.w};
         float m = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
         int cnt = 0;
+        int incl[8];  // bodies in slots 0..k
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const int v = vs[k];
-            if (v < 0) continue;
-            float x, y, z, wm;
-            int sub;
-            if (v < n) {
-                float4 t = nodes[v];
-                x = t.x; y = t.y; z = t.z; wm = t.w;
-                sub = 1;
-            } else {
-                const float* f = reinterpret_cast<const float*>(nodes + v);
-                x = ld_cf(f); y = ld_cf(f + 1); z = ld_cf(f + 2); wm = ld_cf(f + 3);
-                sub = ld_c(cnt_base + (v - n));
+            if (v >= 0) {
+                float x, y, z, wm;
+                int sub;
+                if (v < n) {
+                    float4 t = nodes[v];
+                    x = t.x; y = t.y; z = t.z; wm = t.w;
+                    sub = 1;
+                } else {
+                    const float* f = reinterpret_cast<const float*>(nodes + v);
+                    x = ld_cf(f); y = ld_cf(f + 1); z = ld_cf(f + 2); wm = ld_cf(f + 3);
+                    sub = ld_c(cnt_base + (v - n));
+                }
+                cnt += sub;
+                m = m + wm;
+                cx = fmaf(x, wm, cx);
+                cy = fmaf(y, wm, cy);
+                cz = fmaf(z, wm, cz);
+                cdata[8 * ((size_t)d.cell_off + c) + k] = make_float4(x, y, z, wm);
             }
-            slotoff[8 * ((size_t)d.cell_off + c) + k] = cnt;  // bodies in earlier slots
-            cnt += sub;
-            m = m + wm;
-            cx = fmaf(x, wm, cx);
-            cy = fmaf(y, wm, cy);
-            cz = fmaf(z, wm, cz);
-            cdata[8 * ((size_t)d.cell_off + c) + k] = make_float4(x, y, z, wm);
+            incl[k] = cnt;
         }
+        int4* so = reinterpret_cast<int4*>(slotoff + 8 * ((size_t)d.cell_off + c));
+        so[0] = make_int4(incl[0], incl[1], incl[2], incl[3]);
+        so[1] = make_int4(incl[4], incl[5], incl[6], incl[7]);
         const float inv = 1.0f / m;
         float* o = reinterpret_cast<float*>(nodes + n + c);
         st_cf(o, cx * inv);
@@ -570,17 +575,39 @@ __global__ __launch_bounds__(256) void k_summarize(const SysDesc* __restrict__ s
 // sort: tree-order placement (Burtscher & Pingali's in-order placement, which
 // the reference leaves empty).  A body's rank in depth-first slot order is the
 // sum, along its root-to-leaf path, of the bodies held by earlier slots of each
-// cell on the path; summarize stored those per-slot offsets.  One lane per body
+// cell on the path; summarize stored per-slot counts.  One lane per body
 // re-derives its path (the same octant digits as the build) and adds them up:
 // no waiting, no dependence on the order in which cells were allocated.
+
+// Force order (the vote groups' order): depth-first with each cell's children
+// visited along a 3-D Hilbert curve, so consecutive bodies stay in face-sharing
+// cells.  24 orientations, the root's is 0.  kHPos[s] packs, 3 bits per octant,
+// the octant's place in the visiting order of orientation s; kHNext[s] packs,
+// 5 bits per octant, the child's orientation.  The tables follow the Hilbert
+// index of Skilling ("Programming the Hilbert curve", AIP Conf. Proc. 707, 2004)
+// on the octree's octants 4[y<=c]+2[x<=c]+[z<=c].
+__constant__ unsigned kHPos[24] = {
+    0x07e6a5, 0x1ce715, 0xc78aa3, 0x97e681, 0x8f8ab1, 0x05afa5, 0x1dc395, 0x9d8b91,
+    0x0fc2b5, 0xdec213, 0x9ee611, 0x0caf35, 0x95af81, 0x8eae31, 0xc6ae23, 0xdc8b13,
+    0x458ba7, 0xd7c283, 0x4c8b37, 0x55c387, 0x54e707, 0xd4af03, 0x4ec237, 0x46e627};
+__constant__ unsigned long long kHNext[24] = {
+    0x3148310020ull, 0x41d6a78021ull, 0x8c107109e0ull, 0xa0c4020e6cull, 0x90e3021107ull, 0x081c56b0c5ull,
+    0x5a92541cc6ull, 0x82552398e4ull, 0x2c46b32088ull, 0x7852eb3526ull, 0xbbd413d94dull, 0x0542b621abull,
+    0x9b1ac71683ull, 0xb31adadd6aull, 0xaddce5a925ull, 0x49a44789e1ull, 0x80aef84cf6ull, 0x144159c588ull,
+    0x951e2912d3ull, 0x9cecc8c2b7ull, 0xa52b790e30ull, 0x756952c46bull, 0xb4ecd4ba54ull, 0xbbaf482552ull};
 
 // keys (optional): per force workgroup of the coming force pass, the largest
 // interaction count that its waves' first bodies had in the previous force pass
 // (acc4.w), the dispatch-order key of k_force_order.
+// The same walk gives each body's place in the force order: at every cell, the
+// bodies of the children that come earlier on the cell's Hilbert order precede
+// it.  Both ranks come from the cell's 8 inclusive per-slot counts (one 32-byte
+// read per level).
 __global__ __launch_bounds__(256) void k_sort(const SysDesc* __restrict__ sd, SysState* st,
                                               const BlkWork* __restrict__ work, const float4* __restrict__ node4,
                                               const int* __restrict__ child, const int* __restrict__ slotoff,
-                                              int* sorted, const float4* __restrict__ acc4, int* keys) {
+                                              int* sorted, int* forder,
+                                              const float4* __restrict__ acc4, int* keys) {
     const BlkWork w = work[blockIdx.x];
     const SysDesc d = sd[w.sys];
     SysState* S = st + w.sys;
@@ -593,16 +620,34 @@ __global__ __launch_bounds__(256) void k_sort(const SysDesc* __restrict__ sd, Sy
     const float4 p = node4[d.node_off + b];
     float lx, ly, lz, hx, hy, hz;
     root_box(d.rules, S->box, lx, ly, lz, hx, hy, hz);
-    int node = 0, rank = 0;
+    const bool hilbert = d.rules != 0;  // REFERENCE cells are not boxes in y: force order = tree order
+    int node = 0, rank = 0, hrank = 0, hs = 0;
     for (int lvl = 0; lvl <= kMaxLevel; ++lvl) {
         const int dg = level_digit(d.rules, lvl, p.x, p.y, p.z, lx, ly, lz, hx, hy, hz);
         const int v = C[8 * node + dg];
-        rank += O[8 * node + dg];
+        if (hilbert) {
+            const int4 i0 = reinterpret_cast<const int4*>(O + 8 * node)[0];
+            const int4 i1 = reinterpret_cast<const int4*>(O + 8 * node)[1];
+            const int incl[8] = {i0.x, i0.y, i0.z, i0.w, i1.x, i1.y, i1.z, i1.w};
+            const unsigned hpos = kHPos[hs];
+            const int mine = (hpos >> (3 * dg)) & 7;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int cnt = incl[k] - (k ? incl[k - 1] : 0);
+                rank += k < dg ? cnt : 0;
+                hrank += (int)((hpos >> (3 * k)) & 7) < mine ? cnt : 0;
+            }
+            hs = (int)(kHNext[hs] >> (5 * dg)) & 31;
+        } else {
+            rank += dg ? O[8 * node + dg - 1] : 0;
+            hrank = rank;
+        }
         if (v == b) {
-            if (rank < n) {
-                sorted[d.body_off + rank] = b;
-                if (keys && (rank & (kForceVote - 1)) == 0)
-                    atomicMax(keys + d.grp_first + rank / (kForceVote * kForceWaves),
+            if ((unsigned)rank < (unsigned)n) sorted[d.body_off + rank] = b;
+            if ((unsigned)hrank < (unsigned)n) {
+                forder[d.body_off + hrank] = b;
+                if (keys && (hrank & (kForceVote - 1)) == 0)
+                    atomicMax(keys + d.grp_first + hrank / (kForceVote * kForceWaves),
                               (int)fminf(fmaxf(acc4[d.body_off + b].w, 0.0f), 1e6f));
             }
             return;
@@ -1157,8 +1202,8 @@ void launch_summarize(hipStream_t s, int nblk, const SysDesc* sd, const SysState
     NB_LAUNCH(k_summarize, nblk, 256, s, sd, st, work, node4, child, cparent, cpend, ccount, slotoff, cdata);
 }
 void launch_sort(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work, const float4* node4,
-                 const int* child, const int* slotoff, int* sorted, const float4* acc4, int* keys) {
-    NB_LAUNCH(k_sort, nblk, 256, s, sd, st, work, node4, child, slotoff, sorted, acc4, keys);
+                 const int* child, const int* slotoff, int* sorted, int* forder, const float4* acc4, int* keys) {
+    NB_LAUNCH(k_sort, nblk, 256, s, sd, st, work, node4, child, slotoff, sorted, forder, acc4, keys);
 }
 void launch_force_order(hipStream_t s, const BlkWork* in, int len, int* keys, BlkWork* out) {
     if (len > 0 && len <= kOrderMax) NB_LAUNCH(k_force_order, 1, 1024, s, in, len, keys, out);

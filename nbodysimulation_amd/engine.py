@@ -177,6 +177,14 @@ class NBodySim:
         self._call("nb_export_cells", _ip(lv), _ip(sl), _ip(ct), _fp(com))
         return lv, sl, ct, com.reshape(-1, 4)
 
+    def export_force_order(self):
+        """Body indices in force order (STANDARD rules: tree order with children along a Hilbert
+        curve; REFERENCE rules: the tree order): the order whose 64-runs are the force pass's
+        vote groups and in which force_range counts."""
+        o = np.empty(self.n, np.int32)
+        self._call("nb_export_force_order", _ip(o))
+        return o
+
     def export_dq(self):
         dq = np.empty(L.DQ_LEN, np.float32)
         self._call("nb_export_dq", _fp(dq))

@@ -237,6 +237,97 @@ void orc_summarize(orc_tree* t, const float* pos, const float* mass) {
  * reference's sort stage is empty (kernels/sort.cl is 0 bytes; the intent is
  * noted at kernels/calculate_forces.cl:148); this is Burtscher & Pingali's
  * in-order placement driven by the per-cell counts saturate_tree computes. */
+/* Force order: the bodies in depth-first order with each cell's children visited
+ * along a 3-D Hilbert curve (no counterpart in the reference, whose work-group
+ * vote groups hold unsorted bodies, calculate_forces.cl:84-148).  The engine's
+ * force kernel forms its vote groups from consecutive bodies of this order:
+ * consecutive Hilbert cells share a face, so a group spans less space than a
+ * group of the lexicographic (Z-order) tree order.  State machine: 24
+ * orientations; ORC_H_ORDER[s] lists the octants (4[y<c]+2[x<c]+[z<c]) in visiting
+ * order, ORC_H_NEXT[s][octant] is the child's orientation; the root is state 0.
+ * The tables follow the Hilbert index of Skilling ("Programming the Hilbert
+ * curve", AIP Conf. Proc. 707, 2004) on the octree's cells. */
+static const uint8_t ORC_H_ORDER[24][8] = {
+    {7, 6, 2, 3, 1, 0, 4, 5},
+    {7, 5, 1, 3, 2, 0, 4, 6},
+    {4, 6, 2, 0, 1, 3, 7, 5},
+    {1, 0, 2, 3, 7, 6, 4, 5},
+    {4, 0, 2, 6, 7, 3, 1, 5},
+    {7, 6, 4, 5, 1, 0, 2, 3},
+    {7, 3, 1, 5, 4, 0, 2, 6},
+    {4, 0, 1, 5, 7, 3, 2, 6},
+    {7, 3, 2, 6, 4, 0, 1, 5},
+    {2, 3, 1, 0, 4, 5, 7, 6},
+    {2, 0, 1, 3, 7, 5, 4, 6},
+    {7, 5, 4, 6, 2, 0, 1, 3},
+    {1, 0, 4, 5, 7, 6, 2, 3},
+    {2, 0, 4, 6, 7, 5, 1, 3},
+    {2, 6, 4, 0, 1, 5, 7, 3},
+    {4, 5, 1, 0, 2, 3, 7, 6},
+    {4, 6, 7, 5, 1, 3, 2, 0},
+    {1, 3, 2, 0, 4, 6, 7, 5},
+    {4, 5, 7, 6, 2, 3, 1, 0},
+    {1, 3, 7, 5, 4, 6, 2, 0},
+    {1, 5, 7, 3, 2, 6, 4, 0},
+    {1, 5, 4, 0, 2, 6, 7, 3},
+    {2, 3, 7, 6, 4, 5, 1, 0},
+    {2, 6, 7, 3, 1, 5, 4, 0}
+};
+static const uint8_t ORC_H_NEXT[24][8] = {
+    {0, 1, 0, 2, 3, 4, 5, 6},
+    {1, 1, 0, 15, 10, 11, 7, 8},
+    {0, 15, 2, 2, 7, 8, 16, 17},
+    {12, 19, 3, 4, 0, 2, 3, 20},
+    {7, 8, 4, 4, 16, 17, 3, 18},
+    {5, 6, 12, 13, 5, 14, 0, 1},
+    {6, 6, 7, 8, 5, 9, 10, 11},
+    {4, 7, 6, 7, 18, 10, 9, 16},
+    {8, 4, 8, 6, 11, 3, 17, 5},
+    {6, 9, 13, 22, 14, 9, 1, 15},
+    {13, 10, 22, 7, 1, 10, 15, 23},
+    {11, 13, 8, 12, 11, 1, 21, 0},
+    {3, 20, 5, 14, 12, 13, 12, 19},
+    {10, 11, 23, 21, 13, 13, 12, 22},
+    {5, 9, 10, 11, 14, 14, 23, 21},
+    {1, 15, 2, 15, 4, 18, 6, 9},
+    {22, 7, 19, 16, 15, 23, 2, 16},
+    {8, 12, 17, 19, 21, 0, 17, 2},
+    {19, 22, 4, 18, 2, 15, 20, 18},
+    {23, 21, 16, 17, 12, 22, 19, 19},
+    {16, 17, 3, 18, 23, 21, 20, 20},
+    {11, 3, 17, 5, 21, 20, 21, 14},
+    {20, 18, 14, 9, 13, 22, 19, 22},
+    {18, 10, 9, 16, 20, 23, 14, 23}
+};
+
+void orc_force_order(const orc_tree* t, int32_t* order) {
+    const int n = t->n;
+    if (n == 0) return;
+    if (t->rules == ORC_RULES_REFERENCE) {  /* REFERENCE cells are not boxes in y: tree order */
+        orc_sort(t, order);
+        return;
+    }
+    int stk_node[ORC_MAX_LEVEL + 2], stk_next[ORC_MAX_LEVEL + 2], stk_state[ORC_MAX_LEVEL + 2];
+    int sp = 0, nb = 0;
+    stk_node[0] = 0; stk_next[0] = 0; stk_state[0] = 0;
+    while (sp >= 0) {
+        const int node = stk_node[sp], i = stk_next[sp], s = stk_state[sp];
+        if (i >= 8) { --sp; continue; }
+        stk_next[sp] = i + 1;
+        const int k = ORC_H_ORDER[s][i];
+        const int v = t->child[8 * node + k];
+        if (v < 0) continue;
+        if (v < n) {
+            order[nb++] = v;
+        } else {
+            ++sp;
+            stk_node[sp] = v - n;
+            stk_next[sp] = 0;
+            stk_state[sp] = ORC_H_NEXT[s][k];
+        }
+    }
+}
+
 void orc_sort(const orc_tree* t, int32_t* sorted) {
     orc_export(t, sorted, NULL, NULL, NULL, NULL, NULL, NULL);
 }

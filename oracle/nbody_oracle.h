@@ -58,6 +58,9 @@ int orc_path_of(int rules, const float* p3, const float* box6, int maxlen, uint8
 int orc_build(orc_tree* t, int rules, const float* pos, const float* box6);
 void orc_summarize(orc_tree* t, const float* pos, const float* mass);
 void orc_sort(const orc_tree* t, int32_t* sorted);
+/* The force kernel's vote-group order: STANDARD rules, depth-first with children along a
+ * 3-D Hilbert curve; REFERENCE rules, the tree order of orc_sort. */
+void orc_force_order(const orc_tree* t, int32_t* order);
 int orc_export(const orc_tree* t, int32_t* sorted, uint8_t* paths, int32_t* path_len,
                int32_t* cell_level, int32_t* cell_slot, int32_t* cell_count, float* cell_com);
 void orc_dq_table(int rules, const float* box6, float theta, float eps, float* dq);

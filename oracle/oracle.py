@@ -41,6 +41,7 @@ _SIG = {
     "orc_build": (C.c_int, [_V, C.c_int, _FP, _FP]),
     "orc_summarize": (None, [_V, _FP, _FP]),
     "orc_sort": (None, [_V, _IP]),
+    "orc_force_order": (None, [_V, _IP]),
     "orc_export": (C.c_int, [_V, _IP, _U8P, _IP, _IP, _IP, _IP, _FP]),
     "orc_dq_table": (None, [C.c_int, _FP, C.c_float, C.c_float, _FP]),
     "orc_force_bh": (None, [_V, _FP, _FP, _IP, _FP, C.c_float, C.c_float, C.c_int, C.c_int, _IP, C.c_int, _FP,
@@ -124,10 +125,13 @@ class Tree:
         self.h = self.lib.orc_tree_new(self.n, self.cap)
         self.status = self.lib.orc_build(self.h, rules, fp(self.pos), fp(self.box))
         self.sorted = None
+        self.force_order = None
         if self.status == OK:
             self.lib.orc_summarize(self.h, fp(self.pos), fp(self.mass))
             self.sorted = np.empty(self.n, np.int32)
             self.lib.orc_sort(self.h, ip(self.sorted))
+            self.force_order = np.empty(self.n, np.int32)
+            self.lib.orc_force_order(self.h, ip(self.force_order))
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -158,14 +162,17 @@ class Tree:
     def dq(self, theta, eps):
         return dq_table(self.rules, self.box, theta, eps)
 
-    def force(self, theta, eps, G, vote=64, kahan=False, groups=None):
-        """Barnes-Hut accelerations (n,3); also sum |term| per body and interactions per body."""
+    def force(self, theta, eps, G, vote=64, kahan=False, groups=None, order=None):
+        """Barnes-Hut accelerations (n,3); also sum |term| per body and interactions per body.
+        Vote groups are `vote` consecutive bodies of `order` (default: the force order, as the
+        kernel forms them); `groups` selects group indices of that order."""
         dq = self.dq(theta, eps)
+        order = self.force_order if order is None else np.ascontiguousarray(order, dtype=np.int32)
         acc = np.zeros(3 * self.n, np.float32)
         absacc = np.zeros(self.n, np.float64)
         ninter = np.zeros(self.n, np.int64)
         g = None if groups is None else np.ascontiguousarray(groups, dtype=np.int32)
-        self.lib.orc_force_bh(self.h, fp(self.pos), fp(self.mass), ip(self.sorted), fp(dq), eps, G, vote,
+        self.lib.orc_force_bh(self.h, fp(self.pos), fp(self.mass), ip(order), fp(dq), eps, G, vote,
                               int(kahan), ip(g), 0 if g is None else g.size, fp(acc),
                               absacc.ctypes.data_as(_DP), ninter.ctypes.data_as(_I64P))
         return acc.reshape(-1, 3), absacc, ninter

@@ -5,9 +5,10 @@
 
 Inputs are stored, not regenerated, so a fixture stays valid if the initial-condition
 generators change.  Each case holds the inputs and the oracle's outputs:
-  bh_<rules>_<kind>_<n>: box, thresholds, tree order, octant paths (first max-length
-      digits), cells in preorder (level, slot, count, centre of mass and mass), per-body
-      interactions of the 64-body vote groups, accelerations and sum |term| per body;
+  bh_<rules>_<kind>_<n>: box, thresholds, tree order, force order (the vote groups'
+      Hilbert order), octant paths (first max-length digits), cells in preorder (level,
+      slot, count, centre of mass and mass), per-body interactions of the 64-body vote
+      groups, accelerations and sum |term| per body;
   bf_<law>_<n>: all-pairs accelerations and sum |term| per body;
   step_bh_<rules>_<n>: positions and velocities after 3 Euler Barnes-Hut steps.
 The fixtures pin regressions of the oracle (tests/test_golden.py, CPU) and of the HIP
@@ -44,7 +45,7 @@ def bh_case(rules, kind, n, seed):
     maxlen = int(ex["path_len"].max())
     acc, absacc, ninter = t.force(THETA, EPS, G, vote=VOTE)
     return {"pos": pos, "vel": vel, "mass": mass, "box": t.box, "dq": t.dq(THETA, EPS),
-            "sorted": ex["sorted"], "path_len": ex["path_len"], "paths": ex["paths"][:, :maxlen].copy(),
+            "sorted": ex["sorted"], "force_order": t.force_order, "path_len": ex["path_len"], "paths": ex["paths"][:, :maxlen].copy(),
             "cell_level": ex["cell_level"], "cell_slot": ex["cell_slot"], "cell_count": ex["cell_count"],
             "cell_com": ex["cell_com"], "ninter": ninter.astype(np.int32), "acc": acc, "absacc": absacc,
             "max_depth": np.int32(t.max_depth)}

@@ -38,6 +38,7 @@ def test_oracle_tree_and_force_unchanged(orc, name, c):
     w = c["paths"].shape[1]
     for k in ("sorted", "path_len", "cell_level", "cell_slot", "cell_count"):
         assert np.array_equal(ex[k], c[k]), k
+    assert np.array_equal(t.force_order, c["force_order"])
     assert np.array_equal(ex["paths"][:, :w], c["paths"])
     assert (ex["paths"][:, w:] == 0xFF).all()
     assert ex["cell_com"].tobytes() == c["cell_com"].tobytes()
@@ -76,6 +77,7 @@ def test_gpu_matches_golden_tree_and_force(gpu, name, c):
         box = s.force_pipeline()
         info = s.tree_info()
         srt, paths, plen = s.export_bodies()
+        forder = s.export_force_order()
         lv, sl, ct, com = s.export_cells()
         dq = s.export_dq()
         ninter = s.interactions()
@@ -85,6 +87,7 @@ def test_gpu_matches_golden_tree_and_force(gpu, name, c):
     assert dq.tobytes() == c["dq"].tobytes()
     w = c["paths"].shape[1]
     assert np.array_equal(srt, c["sorted"]) and np.array_equal(plen, c["path_len"])
+    assert np.array_equal(forder, c["force_order"])
     assert np.array_equal(paths[:, :w], c["paths"])
     assert np.array_equal(lv, c["cell_level"]) and np.array_equal(sl, c["cell_slot"])
     assert np.array_equal(ct, c["cell_count"]) and com.tobytes() == c["cell_com"].tobytes()

@@ -61,6 +61,7 @@ def compare_tree(s, t, n):
     assert np.array_equal(plen, ex["path_len"])
     assert np.array_equal(paths, ex["paths"])
     assert np.array_equal(srt, ex["sorted"])
+    assert np.array_equal(s.export_force_order(), t.force_order)
     lv, sl, ct, com = s.export_cells()
     assert np.array_equal(lv, ex["cell_level"])
     assert np.array_equal(sl, ex["cell_slot"])
@@ -262,7 +263,7 @@ def test_full_size_tree_exact(orc, gpu, rules):
         groups = np.random.default_rng(0).choice(n // VG, 64, replace=False).astype(np.int32)
         a_o, absacc, ni = t.force(0.5, 0.01, 1.0, vote=VG, groups=groups)
         _, _, a_g = s.download()
-        members = t.sorted[(groups[:, None] * VG + np.arange(VG)[None, :]).reshape(-1)]
+        members = t.force_order[(groups[:, None] * VG + np.arange(VG)[None, :]).reshape(-1)]
         err = np.linalg.norm(a_g[members] - a_o[members], axis=1)
         assert (err <= ACC_RTOL * absacc[members]).all()
     finally:

@@ -1,6 +1,6 @@
 """Target-range force passes: one GPU's share of a force pass when the targets are
 sharded over G GPUs that each hold the whole tree (rank r computes
-[r*N/G, (r+1)*N/G) of the tree order).  The union of the shares equals the full
+[r*N/G, (r+1)*N/G) of the force order).  The union of the shares equals the full
 pass bit for bit, wherever the cuts fall, and bodies outside every share keep
 their accelerations."""
 import numpy as np
@@ -36,14 +36,14 @@ def test_bh_shares_equal_full_pass(gpu, rules):
         s.upload(pos, vel, mass)
         s.force_pipeline()
         _, _, full = s.download()
-        order = s.export_bodies(paths=False)[0]
+        order = s.export_force_order()
         for cuts in cut_sets(n, 5 + rules):
             pieces = list(zip(cuts[:-1], cuts[1:]))
             left_out = len(pieces) // 2
             print(f"rules={rules} cuts={cuts} left out {pieces[left_out]}", flush=True)
             s.upload(pos, vel, mass)  # clears the accelerations
             tree_up_to_sort(s)
-            assert np.array_equal(s.export_bodies(paths=False)[0], order)
+            assert np.array_equal(s.export_force_order(), order)
             for k, (a, b) in enumerate(pieces):
                 if k != left_out:
                     s.force_range(a, b - a)

@@ -233,6 +233,89 @@ def test_sort_is_lexicographic_path_order(orc, rules):
     assert [keys[i] for i in s] == sorted(keys)
 
 
+def skilling_hilbert_index(q, bits):
+    """3-D Hilbert index of integer coordinates q (m,3) < 2**bits by Skilling's transpose
+    algorithm ("Programming the Hilbert curve", AIP Conf. Proc. 707, 2004): axes to
+    transpose, Gray encode, then interleave the bits with axis 0 most significant."""
+    X = [int(v) for v in q]
+    M = 1 << (bits - 1)
+    Q = M
+    while Q > 1:
+        P = Q - 1
+        for i in range(3):
+            if X[i] & Q:
+                X[0] ^= P
+            else:
+                t = (X[0] ^ X[i]) & P
+                X[0] ^= t
+                X[i] ^= t
+        Q >>= 1
+    for i in range(1, 3):
+        X[i] ^= X[i - 1]
+    t = 0
+    Q = M
+    while Q > 1:
+        if X[2] & Q:
+            t ^= Q - 1
+        Q >>= 1
+    X = [x ^ t for x in X]
+    key = 0
+    for b in range(bits - 1, -1, -1):
+        for i in range(3):
+            key = (key << 1) | ((X[i] >> b) & 1)
+    return key
+
+
+@pytest.mark.parametrize("bits", [2, 3, 4])
+def test_force_order_is_skilling_hilbert_on_a_grid(orc, bits):
+    """STANDARD rules (a cube root, true octants): one body per finest cell of a 2^bits grid;
+    the tree's force order is the order of Skilling's Hilbert index of the integer cell
+    coordinates (x, y, z) as axes (0, 1, 2); the curve starts in the low corner (octant 7)."""
+    g = 1 << bits
+    idx = np.array([(i, j, k) for i in range(g) for j in range(g) for k in range(g)])
+    perm = rng(bits).permutation(len(idx))
+    idx = idx[perm]
+    pos = ((idx + 0.5) / g).astype(np.float32)
+    t = orc.Tree(pos, np.ones(len(pos), np.float32), R_STD, cap=4 * len(pos) + 64)
+    keys = np.array([skilling_hilbert_index(q, bits) for q in idx])
+    assert np.array_equal(t.force_order, np.argsort(keys, kind="stable"))
+    # consecutive bodies of a Hilbert order sit in face-sharing cells
+    d = np.abs(np.diff(idx[t.force_order], axis=0)).sum(1)
+    assert (d == 1).all()
+
+
+@pytest.mark.parametrize("rules", [R_REF, R_STD])
+def test_force_order_keeps_cells_contiguous(orc, rules):
+    """STANDARD: a depth-first order other than the lexicographic one (a permutation in which
+    the bodies of every cell are consecutive).  REFERENCE: the tree order itself (its cells
+    are not boxes in y, so no curve over them keeps neighbours adjacent)."""
+    r = rng(13)
+    n = 3000
+    pos = r.standard_cauchy(size=(n, 3)).clip(-50, 50).astype(np.float32)
+    t = orc.Tree(pos, np.ones(n, np.float32), rules)
+    ex = t.export()
+    fo = t.force_order
+    assert sorted(fo.tolist()) == list(range(n))
+    where = np.empty(n, np.int64)
+    where[fo] = np.arange(n)
+    paths, ln = ex["paths"], ex["path_len"]
+    # every prefix (cell) of every body: its members occupy one run of the force order
+    seen = set()
+    for b in range(0, n, 37):
+        for lv in range(1, ln[b]):
+            pre = bytes(paths[b][:lv])
+            if pre in seen:
+                continue
+            seen.add(pre)
+            member = (ln > lv) & np.all(paths[:, :lv] == paths[b][:lv], axis=1)
+            w = np.sort(where[member])
+            assert w[-1] - w[0] + 1 == len(w)
+    if rules == R_REF:
+        assert np.array_equal(fo, ex["sorted"])
+    else:
+        assert not np.array_equal(fo, ex["sorted"])
+
+
 # ---------------------------------------------------------------- opening thresholds (KATs)
 
 def test_dq_reference_kat(orc):
