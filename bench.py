@@ -239,10 +239,11 @@ def bench_bh(args):
 
 
 # Algorithmic HBM bytes per step of the tree stages and the update (N bodies, C cells used):
-# bounds 12N, reset 32(C+1), build 16N+20C, summarize 16N+72C, sort 4N+40C, update 60N.
+# bounds 12N, reset 32(C+1), build 16N+20C, summarize 16N+72C, sort 24N+64C (positions in,
+# two permutations out; per cell the child slots and the 8 per-slot counts), update 60N.
 STAGE_BYTES = {"bounds": lambda n, c: 12 * n, "reset": lambda n, c: 32 * (c + 1),
                "build": lambda n, c: 16 * n + 20 * c, "summarize": lambda n, c: 16 * n + 72 * c,
-               "sort": lambda n, c: 4 * n + 40 * c, "update": lambda n, c: 60 * n}
+               "sort": lambda n, c: 24 * n + 64 * c, "update": lambda n, c: 60 * n}
 PEAK_HBM_GBS = 8000.0
 
 
@@ -259,7 +260,7 @@ def stage_bandwidth(n, cells, stages, steps):
 
 def target_shares(pos, vel, mass, rules, theta, eps, G, dt, gpus=(2, 4, 8), reps=5):
     """One GPU's share of the force pass when the targets are split over G GPUs that each hold
-    the whole tree: the force stage over [r*N/G, (r+1)*N/G) of the tree order, timed with HIP
+    the whole tree: the force stage over [r*N/G, (r+1)*N/G) of the force order, timed with HIP
     events on the simulation's stream (mean of `reps` launches after one warm-up), for every rank
     r.  The all-gather of positions that such a split needs is not part of it.  The full pass is timed
     the same way; repeated without a new sort it has no dispatch-order keys, so like the shares it
@@ -286,7 +287,7 @@ def target_shares(pos, vel, mass, rules, theta, eps, G, dt, gpus=(2, 4, 8), reps
         hip.hipEventElapsedTime(C.byref(ms), ev0, ev1)
         return ms.value / reps
 
-    out = {"reps": reps, "order": "tree order, rank r takes [r*N/G, (r+1)*N/G)"}
+    out = {"reps": reps, "order": "force order, rank r takes [r*N/G, (r+1)*N/G)"}
     with NBodySim(make_config(n, L.BARNES_HUT, rules=rules, theta=theta, eps=eps, G=G, dt=dt), stream=stream) as sim:
         sim.upload(pos, vel, mass)
         sim.force_pipeline()
