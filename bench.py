@@ -35,6 +35,20 @@ PEAK_FP32_TFLOPS = 157.3
 FLOPS_PER_INTERACTION = 20
 
 
+def host_cpu():
+    """The host CPU the CPU baseline ran on: model name (from /proc/cpuinfo) and logical CPUs."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count()}
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -188,7 +202,7 @@ def bench_bh(args):
     t_f = time.perf_counter()
     cpu_step_s = (t_b - t_a) + (t_c - t_b) + (t_d - t_c) * n / sample + (t_f - t_e)
     if not args.no_cpu_baseline:
-        cpu = {"value": 1.0 / cpu_step_s, "unit": "timesteps/s", "cores": orc.threads(), "kind": "port",
+        cpu = {"value": 1.0 / cpu_step_s, "unit": "timesteps/s", "cores": orc.threads(), "host": host_cpu(), "kind": "port",
                "sample": (f"one step of the same workload on the oracle: bounds, sequential tree build, summarize "
                           f"and sort over all {n} bodies, per-body traversal of {sample} sampled bodies scaled "
                           f"to {n}, Euler update; {cpu_step_s:.2f} s per step")}
@@ -477,7 +491,7 @@ def bench_precision(args):
         t0 = time.perf_counter()
         orc.force_bf(pos, mass, L.LAW_PLUMMER, eps, G, targets=ts)
         t_cpu = (time.perf_counter() - t0) * n / ts.size
-        cpu = {"value": 1.0 / t_cpu, "unit": "timesteps/s", "cores": orc.threads(), "kind": "port",
+        cpu = {"value": 1.0 / t_cpu, "unit": "timesteps/s", "cores": orc.threads(), "host": host_cpu(), "kind": "port",
                "sample": f"oracle brute-force force pass (plain FP32) on 2048 of {n} targets, scaled to {n}; "
                          f"{t_cpu:.2f} s per step"}
     flops = FLOPS_PER_INTERACTION * float(n) * n
@@ -662,7 +676,7 @@ def bench_bf4096(args):
             a, _ = orc.force_bf(p, mass, orc.LAW_REFERENCE_UNIT, 0.01, 1.0)
             v, p = orc.euler(a, v, p, dt)
         t_cpu = (time.perf_counter() - t0) / k
-        cpu = {"value": 1.0 / t_cpu, "unit": "timesteps/s", "cores": orc.threads(), "kind": "port",
+        cpu = {"value": 1.0 / t_cpu, "unit": "timesteps/s", "cores": orc.threads(), "host": host_cpu(), "kind": "port",
                "sample": f"{k} full steps of the same workload on the oracle (all-pairs force + Euler)"}
     r = res["reference_unit"]
     flops = FLOPS_PER_INTERACTION * float(n) * n

@@ -110,7 +110,8 @@ int32_t nb_abi_version(void);
 /* Device description: compute units and name (name may be NULL). */
 nb_status nb_device_info(int32_t* compute_units, char* name, size_t name_len);
 /* Bodies per opening decision of the Barnes-Hut force kernel: the traversal is shared by
- * groups of this many consecutive tree-ordered bodies, and a cell is accepted for the
+ * groups of this many consecutive bodies of the force order (nb_export_force_order), and
+ * a cell is accepted for the
  * group only if it is far enough for every body of it (the reference votes over its
  * work-group, calculate_forces.cl:121). */
 int32_t nb_force_vote_group(void);

@@ -659,12 +659,13 @@ __global__ __launch_bounds__(256) void k_sort(const SysDesc* __restrict__ sd, Sy
 }
 
 // ---------------------------------------------------------------------------
-// force (Barnes-Hut): one wave per 64 consecutive bodies in tree order, one
+// force (Barnes-Hut): one wave per 64 consecutive bodies of the force order, one
 // shared traversal per wave.  A body child is always taken; a cell child at
 // parent level l is taken for the whole wave iff every lane has
 // eps^2+|d|^2 >= dq[l] (the wave-wide form of work_group_all,
 // calculate_forces.cl:122), otherwise pushed.  The stack lives in LDS; node
-// reads are wave-uniform (scalar loads).  acc = G * sum d*m*rsqrt(d2)^3.
+// reads are wave-uniform (scalar loads), and so is every per-child decision,
+// which the build keeps as plain scalar branches.  acc = G * sum d*m*rsqrt(d2)^3.
 
 constexpr int kStack = 8 * kDqLen;
 

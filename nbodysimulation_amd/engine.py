@@ -106,7 +106,7 @@ class NBodySim:
     def collision(self): self._call("nb_stage_collision")
 
     def force_range(self, first: int, count: int):
-        """Barnes-Hut force for the targets [first, first+count) of the tree order only."""
+        """Barnes-Hut force for the targets [first, first+count) of the force order only."""
         self._call("nb_stage_force_range", int(first), int(count))
 
     def brute_force_range(self, first: int, count: int):

@@ -413,7 +413,8 @@ static inline float orc_d2(float dx, float dy, float dz, float e2) {
     return fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, e2)));
 }
 
-/* Barnes-Hut force for vote groups of `vote` consecutive bodies in tree order
+/* Barnes-Hut force for vote groups of `vote` consecutive bodies of `sorted` (the caller
+ * passes the force order, orc_force_order, to emulate the engine's groups)
  * (calculate_force_local, kernels/calculate_forces.cl:84-148).  A body child is
  * always taken; a cell child at parent level l is taken for the whole group iff
  * every member has eps^2+|d|^2 >= dq[l] (the reference votes with
@@ -671,7 +672,8 @@ void orc_energy_momentum_f64(int law, double eps, double G, int n, const float* 
 }
 
 /* Full Barnes-Hut Euler step sequence (CMyApp::Update, src/MyApp.cpp:583-598):
- * bounds -> reset -> build -> summarize -> sort -> force -> update. */
+ * bounds -> reset -> build -> summarize -> sort -> force -> update; the force pass
+ * groups bodies in the force order, as the engine does. */
 int orc_bh_step(int rules, int nsteps, int n, int cap, float theta, float eps, float G, float dt,
                 int vote, int kahan, float* pos, float* vel, const float* mass, float* acc) {
     orc_tree* t = orc_tree_new(n, cap);
@@ -684,7 +686,7 @@ int orc_bh_step(int rules, int nsteps, int n, int cap, float theta, float eps, f
         st = orc_build(t, rules, pos, box);
         if (st) break;
         orc_summarize(t, pos, mass);
-        orc_sort(t, sorted);
+        orc_force_order(t, sorted);  /* the engine's vote groups */
         orc_dq_table(rules, box, theta, eps, dq);
         orc_force_bh(t, pos, mass, sorted, dq, eps, G, vote, kahan, NULL, 0, acc, NULL, NULL);
         orc_euler(n, dt, acc, vel, pos);

@@ -7,7 +7,7 @@ thresholds are bit-exact; so is every opening decision (per-body interaction
 counts equal).  Accelerations differ only through the hardware reciprocal
 square root and summation order, so they are held to
 |a_gpu - a_oracle| <= ACC_RTOL * sum_k |term_k| per body (ACC_RTOL = 2e-5).
-The oracle emulates the kernel's vote groups (VG consecutive tree-ordered bodies).
+The oracle emulates the kernel's vote groups (VG consecutive bodies of the force order).
 """
 import numpy as np
 import pytest
