@@ -698,8 +698,6 @@ __global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __
     int nint = 0;
     int* sk = stk[wave];
     int sp = 0;
-    // lanes 0..7 hold the current cell's slot k and child k's (x,y,z,m);
-    // the next cell's loads are issued before the current one is processed
     sk[0] = 0;
     sp = 1;
     while (sp > 0) {
@@ -708,15 +706,34 @@ __global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __
         const int node = cur >> 7;
         const int lvl = min(cur & 127, kMaxLevel);
         const float dql = dqs[lvl];
-        // wave-uniform scalar loads: the 8 slots and the 8 children's (x,y,z,m),
-        // one contiguous 128-byte block per cell
-        const int4* c4 = reinterpret_cast<const int4*>(C + 8 * node);
-        const int4 s0 = c4[0], s1 = c4[1];
-        const int vs[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
-        const float4* qb = CD + 8 * node;
+        // The cell's 8 slots (32 bytes) and its children's (x,y,z,m) (one 128-byte
+        // block) in one batch of wave-uniform scalar loads with a single wait, so that
+        // a popped cell costs one scalar-memory round trip.  Left to itself the
+        // compiler sinks each child's load into the branch that uses it and waits up to
+        // three times per cell, which lengthens the walk of the longest waves.  The
+        // wait is inside the asm block: no load is in flight when it ends.
+        typedef int v8i __attribute__((ext_vector_type(8)));
+        typedef float v16f __attribute__((ext_vector_type(16)));
+        v8i sv;
+        v16f q0, q1;
+        asm volatile(
+            "s_load_dwordx8 %0, %3, 0x0\n\t"
+            "s_load_dwordx16 %1, %4, 0x0\n\t"
+            "s_load_dwordx16 %2, %4, 0x40\n\t"
+            "s_waitcnt lgkmcnt(0)"
+            : "=&s"(sv), "=&s"(q0), "=&s"(q1)
+            : "s"(C + 8 * node), "s"(CD + 8 * node)
+            : "memory");
+        const int vs[8] = {sv[0], sv[1], sv[2], sv[3], sv[4], sv[5], sv[6], sv[7]};
         float4 qs[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) qs[k] = qb[k];
+        qs[0] = make_float4(q0[0], q0[1], q0[2], q0[3]);
+        qs[1] = make_float4(q0[4], q0[5], q0[6], q0[7]);
+        qs[2] = make_float4(q0[8], q0[9], q0[10], q0[11]);
+        qs[3] = make_float4(q0[12], q0[13], q0[14], q0[15]);
+        qs[4] = make_float4(q1[0], q1[1], q1[2], q1[3]);
+        qs[5] = make_float4(q1[4], q1[5], q1[6], q1[7]);
+        qs[6] = make_float4(q1[8], q1[9], q1[10], q1[11]);
+        qs[7] = make_float4(q1[12], q1[13], q1[14], q1[15]);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const int v = vs[k];
