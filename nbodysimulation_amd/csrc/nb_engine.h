@@ -30,11 +30,12 @@ void launch_summarize(hipStream_t s, int nblk, const SysDesc* sd, const SysState
                       float4* node4, const int* child, const int* cparent, int* cpend, int* ccount, int* slotoff,
                       float4* cdata);
 void launch_sort(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work, const float4* node4,
-                 const int* child, const int* slotoff, int* sorted, int* forder, const float4* acc4, int* keys);
+                 const int* child, const int* slotoff, int* sorted, int* forder, const float4* acc4, int* keys,
+                 const int* sorder);
 void launch_force_order(hipStream_t s, const BlkWork* in, int len, int* keys, BlkWork* out);
-void launch_force_bh(hipStream_t s, int nblk, bool kahan, const SysDesc* sd, const SysState* st,
+void launch_force_bh(hipStream_t s, int nblk, bool kahan, const SysDesc* sd, SysState* st,
                      const BlkWork* work, const float4* node4, const int* child, const float4* cdata,
-                     const int* sorted, float4* acc4);
+                     const int* order, float4* acc4, int* sorder);
 // mode >= 0: also integrate (k_update's mode 0 or 1); mode 0 writes the moved
 // positions to out4 (same layout as node4), never in place
 void launch_brute(hipStream_t s, int nblk, int law, bool kahan, int tb, const SysDesc* sd, const SysState* st,
@@ -90,7 +91,7 @@ struct Capacity {
 
 // Bytes of device memory per node, body and cell in the arena layout.
 constexpr size_t kBytesPerNode = 16;                     // node4
-constexpr size_t kBytesPerBody = 16 + 16 + 4 + 4;        // vel4, acc4, sorted, forder
+constexpr size_t kBytesPerBody = 16 + 16 + 4 + 4 + 4;    // vel4, acc4, sorted, forder, sorder
 constexpr size_t kBytesPerCell = 32 + 4 * 3 + 32 + 128;  // child, parent/pend/count, slotoff, cdata
 
 struct Engine {
@@ -115,7 +116,7 @@ struct Engine {
     void* arena = nullptr;
     size_t arena_bytes = 0;
     float4 *node4 = nullptr, *vel4 = nullptr, *acc4 = nullptr, *cdata = nullptr;
-    int *sorted = nullptr, *forder = nullptr, *child = nullptr, *cparent = nullptr, *cpend = nullptr, *ccount = nullptr,
+    int *sorted = nullptr, *forder = nullptr, *sorder = nullptr, *child = nullptr, *cparent = nullptr, *cpend = nullptr, *ccount = nullptr,
         *slotoff = nullptr;
     SysDesc* d_desc = nullptr;
     SysState* d_state = nullptr;

@@ -144,6 +144,7 @@ nb_status Engine::reserve(const Capacity& c, int law_, bool kahan_, void* user_s
         {(void**)&acc4, sizeof(float4) * nb},
         {(void**)&sorted, sizeof(int) * nb},
         {(void**)&forder, sizeof(int) * nb},
+        {(void**)&sorder, sizeof(int) * nb},
         {(void**)&child, sizeof(int) * 8 * nc},
         {(void**)&cparent, sizeof(int) * nc},
         {(void**)&cpend, sizeof(int) * nc},
@@ -172,6 +173,7 @@ nb_status Engine::reserve(const Capacity& c, int law_, bool kahan_, void* user_s
     NB_HIP(hipMemsetAsync(acc4, 0, sizeof(float4) * nb, stream));
     NB_HIP(hipMemsetAsync(sorted, 0, sizeof(int) * nb, stream));
     NB_HIP(hipMemsetAsync(forder, 0, sizeof(int) * nb, stream));
+    NB_HIP(hipMemsetAsync(sorder, 0, sizeof(int) * nb, stream));
     NB_HIP(hipMemsetAsync(child, 0xff, sizeof(int) * 8 * nc, stream));
     NB_HIP(hipMemsetAsync(cpend, 0, sizeof(int) * nc, stream));
     NB_HIP(hipMemsetAsync(ccount, 0, sizeof(int) * nc, stream));
@@ -340,13 +342,13 @@ void Engine::summarize() {
 void Engine::sort() {
     TraceRange tr("nb:sort");
     launch_sort(stream, tab_len[T_BODY], d_desc, d_state, tab[T_BODY], node4, child, slotoff, sorted, forder,
-                acc4, lpt ? gkey : nullptr);
+                acc4, lpt ? gkey : nullptr, sorder);
 }
 void Engine::force() {
     TraceRange tr("nb:force");
     if (lpt) launch_force_order(stream, tab[T_GROUPS], tab_len[T_GROUPS], gkey, tab[T_FORDER]);
     launch_force_bh(stream, tab_len[T_GROUPS], kahan, d_desc, d_state, tab[lpt ? T_FORDER : T_GROUPS], node4, child,
-                    cdata, forder, acc4);
+                    cdata, forder, acc4, sorder);
 }
 void Engine::force_range(int sys, int lo, int hi) {
     TraceRange tr("nb:force_range");
@@ -355,7 +357,7 @@ void Engine::force_range(int sys, int lo, int hi) {
     const int stop = (int)std::min<long long>(hdesc[sys].n, ((long long)hi + kForceVote - 1) / kForceVote * kForceVote);
     const int nblk = (stop - start + per - 1) / per;
     launch_range_table(stream, tab[T_RANGE], nblk, sys, start, stop, lo, hi, per);
-    launch_force_bh(stream, nblk, kahan, d_desc, d_state, tab[T_RANGE], node4, child, cdata, forder, acc4);
+    launch_force_bh(stream, nblk, kahan, d_desc, d_state, tab[T_RANGE], node4, child, cdata, forder, acc4, nullptr);
 }
 void Engine::brute_range(int sys, int lo, int hi) {
     TraceRange tr("nb:brute_force_range");

@@ -13,6 +13,8 @@
 //   sorted int[sum n_s]              local body ids in tree order (children in slot order)
 //   forder int[sum n_s]              local body ids in force order (children along a Hilbert
 //                                    curve); the force pass's vote groups are its 64-runs
+//   sorder int[sum n_s]              the order in which the sort visits the bodies: the force
+//                                    order of the last full force pass (identity before one)
 //   child  int[8 * sum cap_s]        per cell 8 slots: -1 empty, -2 locked,
 //                                    [0,n) body, n+ci cell ci (local numbering)
 //   cparent, cpend, ccount           int[sum cap_s]   parent, unfinished child cells, bodies below
@@ -74,7 +76,8 @@ struct SysState {
     int step;
     int bq;         // next entry of the build's last-phase body queue
     float dq[kDqLen];
-    int pad1[3];
+    int order_ok;   // 1: sorder holds a permutation of the bodies (the last full force pass's order)
+    int pad1[2];
 };
 
 // One workgroup's share of one system: items [begin, end).  `trim` is only set

@@ -607,14 +607,18 @@ __global__ __launch_bounds__(256) void k_sort(const SysDesc* __restrict__ sd, Sy
                                               const BlkWork* __restrict__ work, const float4* __restrict__ node4,
                                               const int* __restrict__ child, const int* __restrict__ slotoff,
                                               int* sorted, int* forder,
-                                              const float4* __restrict__ acc4, int* keys) {
+                                              const float4* __restrict__ acc4, int* keys,
+                                              const int* __restrict__ sorder) {
     const BlkWork w = work[blockIdx.x];
     const SysDesc d = sd[w.sys];
     SysState* S = st + w.sys;
     if (S->status) return;
-    const int b = w.begin + threadIdx.x;
-    if (b >= w.end) return;
+    const int t = w.begin + threadIdx.x;
+    if (t >= w.end) return;
     const int n = d.n;
+    // lanes take the bodies in the last force pass's order, so that the lanes of a wave
+    // walk nearly the same path (the ranks do not depend on which lane takes which body)
+    const int b = S->order_ok ? min(max(sorder[d.body_off + t], 0), n - 1) : t;
     const int* C = child + 8 * (size_t)d.cell_off;
     const int* O = slotoff + 8 * (size_t)d.cell_off;
     const float4 p = node4[d.node_off + b];
@@ -669,12 +673,14 @@ __global__ __launch_bounds__(256) void k_sort(const SysDesc* __restrict__ sd, Sy
 
 constexpr int kStack = 8 * kDqLen;
 
+// sorder (full passes only, else NULL): receives the force order, the order in
+// which the next sort visits the bodies.
 template <bool KAHAN>
-__global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __restrict__ sd, const SysState* __restrict__ st,
+__global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __restrict__ sd, SysState* st,
                                                   const BlkWork* __restrict__ work,
                                                   const float4* __restrict__ node4, const int* __restrict__ child,
                                                   const float4* __restrict__ cdata, const int* __restrict__ sorted,
-                                                  float4* __restrict__ acc4) {
+                                                  float4* __restrict__ acc4, int* __restrict__ sorder) {
     __shared__ int stk[kForceWaves][kStack];
     __shared__ float dqs[kDqLen];
     const BlkWork w = work[blockIdx.x];
@@ -766,6 +772,10 @@ __global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __
     if (valid && idx >= w.begin + (w.trim & 0xffff) && idx < w.end - (w.trim >> 16)) {
         if (KAHAN) { ax -= kx; ay -= ky; az -= kz; }
         acc4[d.body_off + b] = make_float4(ax * d.G, ay * d.G, az * d.G, (float)nint);
+    }
+    if (sorder && valid) {  // a full pass covers every body of the system once
+        sorder[d.body_off + idx] = b;
+        if (idx == 0) st[w.sys].order_ok = 1;
     }
 }
 
@@ -1220,19 +1230,20 @@ void launch_summarize(hipStream_t s, int nblk, const SysDesc* sd, const SysState
     NB_LAUNCH(k_summarize, nblk, 256, s, sd, st, work, node4, child, cparent, cpend, ccount, slotoff, cdata);
 }
 void launch_sort(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work, const float4* node4,
-                 const int* child, const int* slotoff, int* sorted, int* forder, const float4* acc4, int* keys) {
-    NB_LAUNCH(k_sort, nblk, 256, s, sd, st, work, node4, child, slotoff, sorted, forder, acc4, keys);
+                 const int* child, const int* slotoff, int* sorted, int* forder, const float4* acc4, int* keys,
+                 const int* sorder) {
+    NB_LAUNCH(k_sort, nblk, 256, s, sd, st, work, node4, child, slotoff, sorted, forder, acc4, keys, sorder);
 }
 void launch_force_order(hipStream_t s, const BlkWork* in, int len, int* keys, BlkWork* out) {
     if (len > 0 && len <= kOrderMax) NB_LAUNCH(k_force_order, 1, 1024, s, in, len, keys, out);
 }
-void launch_force_bh(hipStream_t s, int nblk, bool kahan, const SysDesc* sd, const SysState* st,
+void launch_force_bh(hipStream_t s, int nblk, bool kahan, const SysDesc* sd, SysState* st,
                      const BlkWork* work, const float4* node4, const int* child, const float4* cdata,
-                     const int* sorted, float4* acc4) {
+                     const int* order, float4* acc4, int* sorder) {
     if (kahan)
-        NB_LAUNCH(k_force_bh<true>, nblk, 64 * kForceWaves, s, sd, st, work, node4, child, cdata, sorted, acc4);
+        NB_LAUNCH(k_force_bh<true>, nblk, 64 * kForceWaves, s, sd, st, work, node4, child, cdata, order, acc4, sorder);
     else
-        NB_LAUNCH(k_force_bh<false>, nblk, 64 * kForceWaves, s, sd, st, work, node4, child, cdata, sorted, acc4);
+        NB_LAUNCH(k_force_bh<false>, nblk, 64 * kForceWaves, s, sd, st, work, node4, child, cdata, order, acc4, sorder);
 }
 void launch_range_table(hipStream_t s, BlkWork* out, int nblk, int sys, int start, int stop, int lo, int hi,
                         int per) {

@@ -168,6 +168,30 @@ def test_step_matches_oracle_and_is_deterministic(orc, gpu, rules):
     assert np.abs(p_g - p_o).max() <= 1e-6 * (1 + np.abs(p_o).max())
 
 
+@pytest.mark.parametrize("rules", [0, 1])
+def test_tree_exact_after_steps_and_reupload(orc, gpu, rules):
+    """After full force passes the sort visits the bodies in the last force order (not index
+    order); the tree, both permutations and every opening decision stay those of the oracle
+    on the current positions, also after new positions of the same N are uploaded."""
+    n = 30000
+    pos, vel, mass = ic.plummer(n, 17)
+    other = ic.uniform_ball(n, 18)
+    with NBodySim(make_config(n, L.BARNES_HUT, rules=rules, law=L.LAW_PLUMMER)) as s:
+        s.upload(pos, vel, mass)
+        s.step(3)
+        for p_now, m_now in ((s.download()[0], mass), (other[0], other[2])):
+            if p_now is other[0]:
+                s.upload(*other)  # same N, unrelated positions: the old visiting order still applies
+            say(f"case rules={rules} n={n} after steps, reupload={p_now is other[0]}")
+            s.force_pipeline()
+            t = orc.Tree(p_now, m_now, rules, cap=s.tree_info()["capacity"])
+            assert t.status == orc.OK
+            compare_tree(s, t, n)
+            _, _, ni = t.force(0.5, 0.01, 1.0, vote=VG)
+            assert s.interactions() == int(ni.sum())
+            s.step(2)
+
+
 def test_leapfrog_kdk(orc, gpu):
     n = 4096
     pos, vel, mass = ic.plummer(n, 8)
