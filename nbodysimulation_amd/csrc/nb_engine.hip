@@ -17,7 +17,6 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 #define NB_PHASE_STRIDES 1024, 32, 1
 #endif
 static constexpr int kPhaseStride[Engine::kBuildPhases] = {NB_PHASE_STRIDES};
-static constexpr int kBuildBlock = 1024;  // k_build workgroup size
 static constexpr int kBoundsPerBlock = 2048;
 
 #define NB_HIP(call)                                                                   \
@@ -133,7 +132,8 @@ nb_status Engine::reserve(const Capacity& c, int law_, bool kahan_, void* user_s
     tab_cap[T_RANGE] = blocks_bound(c.bodies, 16, S);  // the smallest share per block of either force kernel
     tab_cap[T_FORDER] = tab_cap[T_GROUPS];
     for (int ph = 0; ph < kBuildPhases; ++ph)
-        tab_cap[T_PHASE0 + ph] = blocks_bound((c.bodies + kPhaseStride[ph] - 1) / kPhaseStride[ph], kBuildBlock, S);
+        tab_cap[T_PHASE0 + ph] =
+            blocks_bound((c.bodies + kPhaseStride[ph] - 1) / kPhaseStride[ph], build_block(kPhaseStride[ph]), S);
 
     struct Region { void** dst; size_t bytes; };
     const size_t nn = (size_t)std::max<long long>(c.nodes, 1), nb = (size_t)std::max<long long>(c.bodies, 1),
@@ -240,7 +240,8 @@ nb_status Engine::activate(const std::vector<Slot>& slots) {
             d.grp_first = (int)t[T_GROUPS].size();
             add_ranges(t[T_GROUPS], s, d.n, kForceVote * kForceWaves);
             for (int ph = 0; ph < kBuildPhases; ++ph)
-                add_ranges(t[T_PHASE0 + ph], s, (d.n + kPhaseStride[ph] - 1) / kPhaseStride[ph], kBuildBlock);
+                add_ranges(t[T_PHASE0 + ph], s, (d.n + kPhaseStride[ph] - 1) / kPhaseStride[ph],
+                           build_block(kPhaseStride[ph]));
         }
         if (!bh && d.n > 0) {
             any_bf = true;

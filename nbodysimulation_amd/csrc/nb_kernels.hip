@@ -294,9 +294,8 @@ __device__ __forceinline__ int wave_incl_scan(int x) {
     return x;
 }
 
-constexpr int kBuildBlock = 1024;  // lanes sharing one allocation atomic per round
-
-__global__ __launch_bounds__(kBuildBlock) void k_build(const SysDesc* __restrict__ sd, SysState* st,
+template <int BLK>  // workgroup size: build_block(stride)
+__global__ __launch_bounds__(BLK) void k_build(const SysDesc* __restrict__ sd, SysState* st,
                                                const BlkWork* __restrict__ work,
                                                const float4* __restrict__ node4, int* child, int* cparent,
                                                int* cpend, int stride, int exclude) {
@@ -337,7 +336,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_build(const SysDesc* __restrict
         fresh = false;
         active = true;
     };
-    constexpr int kWaves = kBuildBlock / 64;
+    constexpr int kWaves = BLK / 64;
     __shared__ int s_wsum[kWaves];
     __shared__ int s_widle[kWaves];
     __shared__ int s_base, s_qbase;
@@ -354,7 +353,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_build(const SysDesc* __restrict
         if (lane == 0) s_widle[wave] = 64;
         __syncthreads();
         if (threadIdx.x == 0) {
-            s_qbase = atomicAdd(&S->bq, kBuildBlock);
+            s_qbase = atomicAdd(&S->bq, BLK);
             drained = s_qbase >= nq;
         }
         __syncthreads();
@@ -1222,7 +1221,12 @@ void launch_tree_init(hipStream_t s, int nsys, const int* slots, SysState* st) {
 }
 void launch_build(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
                   const float4* node4, int* child, int* cparent, int* cpend, int stride, int exclude) {
-    NB_LAUNCH(k_build, nblk, kBuildBlock, s, sd, st, work, node4, child, cparent, cpend, stride, exclude);
+    if (build_block(stride) == kBuildBlockStrided)
+        NB_LAUNCH(k_build<kBuildBlockStrided>, nblk, kBuildBlockStrided, s, sd, st, work, node4, child, cparent, cpend,
+                  stride, exclude);
+    else
+        NB_LAUNCH(k_build<kBuildBlock>, nblk, kBuildBlock, s, sd, st, work, node4, child, cparent, cpend, stride,
+                  exclude);
 }
 void launch_summarize(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
                       float4* node4, const int* child, const int* cparent, int* cpend, int* ccount, int* slotoff,
