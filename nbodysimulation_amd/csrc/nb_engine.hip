@@ -239,9 +239,15 @@ nb_status Engine::activate(const std::vector<Slot>& slots) {
             add_ranges(t[T_CELLS], s, d.cap, 256);
             d.grp_first = (int)t[T_GROUPS].size();
             add_ranges(t[T_GROUPS], s, d.n, kForceVote * kForceWaves);
-            for (int ph = 0; ph < kBuildPhases; ++ph)
-                add_ranges(t[T_PHASE0 + ph], s, (d.n + kPhaseStride[ph] - 1) / kPhaseStride[ph],
-                           build_block(kPhaseStride[ph]));
+            for (int ph = 0; ph < kBuildPhases; ++ph) {
+                const int blk = build_block(kPhaseStride[ph]);
+                long long items = (d.n + kPhaseStride[ph] - 1) / kPhaseStride[ph];
+                // the queued last phase takes its bodies from the system's queue, so its
+                // workgroup count is free: 5/8 of the CUs' worth of workgroups contend less
+                // for the same slots than one per 1024 bodies (1M: build 0.42 -> 0.35 ms)
+                if (kPhaseStride[ph] == 1) items = std::min<long long>(items, (long long)std::max(1, cus * 5 / 8) * blk);
+                add_ranges(t[T_PHASE0 + ph], s, (int)items, blk);
+            }
         }
         if (!bh && d.n > 0) {
             any_bf = true;
