@@ -12,9 +12,9 @@ const char* last_error() { return g_err.c_str(); }
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-// Insertion phases of the build: every 1024th body, then every 32nd, then the rest.
+// Insertion phases of the build: every 4096th body, then every 64th, then the rest.
 #ifndef NB_PHASE_STRIDES
-#define NB_PHASE_STRIDES 1024, 32, 1
+#define NB_PHASE_STRIDES 4096, 64, 1
 #endif
 static constexpr int kPhaseStride[Engine::kBuildPhases] = {NB_PHASE_STRIDES};
 static constexpr int kBoundsPerBlock = 2048;

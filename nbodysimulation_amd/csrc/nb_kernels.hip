@@ -272,8 +272,8 @@ __global__ void k_tree_init(int nsys, const int* __restrict__ slots, SysState* s
 // touched once per workgroup and round instead of once per split), each lock holder writes its chain
 // with coherent stores, waits for them to complete (s_waitcnt) and publishes
 // the chain by storing the first cell into the locked slot.
-// The build runs in phases over strided subsets of the bodies (every 1024th,
-// then every 32nd, then the rest): the top levels of the tree form with little
+// The build runs in phases over strided subsets of the bodies (every 4096th,
+// then every 64th, then the rest): the top levels of the tree form with little
 // contention, later phases descend through them.  The topology does not depend
 // on insertion order.
 // The descent reads slots with L2-cacheable loads.  A cached value can only be
