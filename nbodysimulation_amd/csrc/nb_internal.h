@@ -30,10 +30,13 @@ constexpr int kDqLen = kMaxLevel + 1;  // opening thresholds per level
 constexpr int kBfWaves = 16;    // brute-force workgroup: 16 waves of 64
 // Build workgroups: the queued last phase shares one allocation atomic per round among
 // 1024 lanes; the strided skeleton phases (few bodies, latency-bound rounds) spread
-// over more CUs in workgroups of 256 (measured at 1M: build 0.42 -> 0.40 ms).
+// over more CUs in smaller workgroups: 256 lanes, and one wave for the sparsest phase.
 constexpr int kBuildBlock = 1024;
 constexpr int kBuildBlockStrided = 256;
-constexpr int build_block(int stride) { return stride > 1 ? kBuildBlockStrided : kBuildBlock; }
+constexpr int kBuildBlockSparse = 64;
+constexpr int build_block(int stride) {
+    return stride > 64 ? kBuildBlockSparse : stride > 1 ? kBuildBlockStrided : kBuildBlock;
+}
 constexpr int kBfTileStride = 4 * 17;  // a wave's LDS tile: up to 4 parts of 16 sources, rows padded
 #ifndef NB_FORCE_WAVES
 #define NB_FORCE_WAVES 4

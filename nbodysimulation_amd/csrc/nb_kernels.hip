@@ -1221,7 +1221,10 @@ void launch_tree_init(hipStream_t s, int nsys, const int* slots, SysState* st) {
 }
 void launch_build(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
                   const float4* node4, int* child, int* cparent, int* cpend, int stride, int exclude) {
-    if (build_block(stride) == kBuildBlockStrided)
+    if (build_block(stride) == kBuildBlockSparse)
+        NB_LAUNCH(k_build<kBuildBlockSparse>, nblk, kBuildBlockSparse, s, sd, st, work, node4, child, cparent, cpend,
+                  stride, exclude);
+    else if (build_block(stride) == kBuildBlockStrided)
         NB_LAUNCH(k_build<kBuildBlockStrided>, nblk, kBuildBlockStrided, s, sd, st, work, node4, child, cparent, cpend,
                   stride, exclude);
     else
