@@ -274,9 +274,9 @@ nb_status Engine::activate(const std::vector<Slot>& slots) {
         if (!t[k].empty())
             NB_HIP(hipMemcpyAsync(tab[k], t[k].data(), sizeof(BlkWork) * t[k].size(), hipMemcpyHostToDevice, stream));
     }
-    // longest-first dispatch only where a launch holds at most two waves per SIMD slot
+    // longest-first dispatch only where a launch holds at most four waves per SIMD slot
     // (8 waves x 4 SIMDs per CU): with more, the last workgroups matter less than locality
-    lpt = tab_len[T_GROUPS] <= kOrderMax && (long long)tab_len[T_GROUPS] * kForceWaves <= 2LL * 32 * cus;
+    lpt = tab_len[T_GROUPS] <= kOrderMax && (long long)tab_len[T_GROUPS] * kForceWaves <= 4LL * 32 * cus;
     NB_HIP(hipMemcpyAsync(d_desc, hdesc.data(), sizeof(SysDesc) * capc.systems, hipMemcpyHostToDevice, stream));
     if (!ids.empty()) NB_HIP(hipMemcpyAsync(d_slots, ids.data(), sizeof(int) * ids.size(), hipMemcpyHostToDevice, stream));
     NB_HIP(hipStreamSynchronize(stream));
