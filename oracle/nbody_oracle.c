@@ -6,7 +6,8 @@
  *
  * Pinning (what checks each function against something other than itself) is
  * in tests/test_oracle.py: closed forms, brute-force path enumeration on tiny
- * inputs, FP64 sums, symmetry, invariants.
+ * inputs, FP64 sums, symmetry, invariants; orc_force_order against Skilling's
+ * Hilbert index computed independently in the test.
  */
 #include "nbody_oracle.h"
 
