@@ -34,7 +34,7 @@ struct nb_sim {
     hipGraph_t graph_def = nullptr;
     hipGraphExec_t fgraph[2] = {};    // fused brute-force steps: one step, two steps
     hipGraph_t fgraph_def[2] = {};
-    float4* pos_b = nullptr;          // second position buffer of fused Euler steps
+    float4* pos_b = nullptr;          // second position buffer of fused Euler steps (allocated at create)
     std::vector<hipEvent_t> ev;  // kEvPerStep per timed step
     float stage_ms[NB_NUM_STAGES] = {};
     double* energy_buf = nullptr;  // per-workgroup partial sums + result of nb_sim_energy
@@ -159,6 +159,19 @@ static SysSpec spec_of(const nb_config* c, int cap) {
     return s;
 }
 
+static bool is_bh(const nb_sim* s) { return s->cfg.algorithm == NB_BARNES_HUT; }
+
+// Untimed brute-force steps of a small system without collision fuse the
+// integrator into the brute-force kernel (one launch per Euler step instead of
+// two).  Euler moves positions, which the kernel still reads as sources, so
+// fused Euler steps alternate between node4 and pos_b: two steps return to
+// node4, a single step copies back with a kernel.  Leapfrog's opening kick and
+// drift stay a launch of their own; the closing kick is fused.  Results are
+// bit-identical to the separate launches.
+static bool fusable(const nb_sim* sim) {
+    return !is_bh(sim) && !sim->cfg.collision && sim->n > 0 && sim->n <= nb::kBfFuseMax;
+}
+
 static void drop_graph(nb_sim* sim) {
     if (sim->graph) (void)hipGraphExecDestroy(sim->graph);
     if (sim->graph_def) (void)hipGraphDestroy(sim->graph_def);
@@ -189,6 +202,16 @@ nb_status nb_sim_create(const nb_config* cfg, void* stream, nb_sim** out) {
                    ? (cfg->cell_capacity > 0 ? cfg->cell_capacity : nb_reference_cell_capacity(sim->n, cus))
                    : 0;
     st = sim->eng.init({spec_of(cfg, sim->cap)}, stream);
+    // the fused Euler steps' second position buffer exists for the simulation's whole
+    // life, so that stepping calls never allocate
+    if (!st && fusable(sim) && cfg->method == NB_EULER) {
+        const hipError_t e = hipMalloc(&sim->pos_b, sizeof(float4) * (size_t)sim->n);
+        if (e != hipSuccess) {
+            sim->pos_b = nullptr;
+            st = fail(e == hipErrorOutOfMemory ? NB_E_OUT_OF_MEMORY : NB_E_HIP,
+                      std::string("hipMalloc(second position buffer): ") + hipGetErrorString(e));
+        }
+    }
     if (st) {
         delete sim;
         return st;
@@ -285,8 +308,6 @@ nb_status nb_sim_clear_status(nb_sim* sim) {
     NB_HIPC(hipMemset(&sim->eng.d_state[0].status, 0, sizeof(int)));
     return NB_OK;
 }
-
-static bool is_bh(const nb_sim* s) { return s->cfg.algorithm == NB_BARNES_HUT; }
 
 nb_status nb_sim_set_cell_capacity(nb_sim* sim, int32_t cells) {
     if (!sim) return fail(NB_E_INVALID_ARG, "NULL sim");
@@ -526,17 +547,6 @@ static nb_status read_stage_times(nb_sim* sim, int nsteps) {
     return NB_OK;
 }
 
-// Untimed brute-force steps of a small system without collision fuse the
-// integrator into the brute-force kernel (one launch per Euler step instead of
-// two).  Euler moves positions, which the kernel still reads as sources, so
-// fused Euler steps alternate between node4 and pos_b: two steps return to
-// node4, a single step copies back.  Leapfrog's opening kick and drift stay a
-// launch of their own; the closing kick is fused.  Results are bit-identical to
-// the separate launches.
-static bool fusable(const nb_sim* sim) {
-    return !is_bh(sim) && !sim->cfg.collision && sim->n > 0 && sim->n <= nb::kBfFuseMax;
-}
-
 static void enqueue_fused(nb_sim* sim, int k) {
     Engine& g = sim->eng;
     if (sim->cfg.method == NB_LEAPFROG) {
@@ -550,8 +560,7 @@ static void enqueue_fused(nb_sim* sim, int k) {
     if (k == 2)
         g.brute_step(0, sim->pos_b, g.node4);
     else
-        (void)hipMemcpyAsync(g.node4, sim->pos_b, sizeof(float4) * (size_t)sim->n, hipMemcpyDeviceToDevice,
-                             g.stream);
+        nb::launch_copy4(g.stream, sim->n, sim->pos_b, g.node4);
 }
 
 // k = 0: one plain step; k = 1, 2: that many fused steps
@@ -562,9 +571,14 @@ static nb_status build_graph(nb_sim* sim, int k, hipGraph_t* def, hipGraphExec_t
         enqueue_step(sim, nullptr);
     else
         enqueue_fused(sim, k);
+    const hipError_t le = hipGetLastError();  // a launch that failed to be recorded
     hipGraph_t g = nullptr;
     hipError_t ce = hipStreamEndCapture(e.stream, &g);
-    if (ce != hipSuccess) return fail(NB_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+    if (ce == hipSuccess && le != hipSuccess) ce = le;
+    if (ce != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(NB_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+    }
     hipGraphExec_t ex = nullptr;
     ce = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     if (ce != hipSuccess) {
@@ -580,7 +594,7 @@ static nb_status build_graph(nb_sim* sim, int k, hipGraph_t* def, hipGraphExec_t
 static nb_status enqueue_steps(nb_sim* sim, int nsteps) {
     Engine& e = sim->eng;
     if (fusable(sim)) {
-        if (sim->cfg.method == NB_EULER && !sim->pos_b) NB_HIPC(hipMalloc(&sim->pos_b, sizeof(float4) * sim->n));
+        if (sim->cfg.method == NB_EULER && !sim->pos_b) return fail(NB_E_STATE, "no second position buffer");
         if (sim->cfg.use_graph) {
             for (int k = 1; k <= 2; ++k)
                 if (!sim->fgraph[k - 1]) {

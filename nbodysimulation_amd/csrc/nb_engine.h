@@ -45,6 +45,7 @@ void launch_range_table(hipStream_t s, BlkWork* out, int nblk, int sys, int star
 // all = false: only the systems whose SysDesc asks for the collision stage
 void launch_collision(hipStream_t s, int nblk, bool all, const SysDesc* sd, const SysState* st, const BlkWork* work,
                       const float4* node4, float4* vel4);
+void launch_copy4(hipStream_t s, int n, const float4* src, float4* dst);
 void launch_vertices(hipStream_t s, int n, const float4* node4, float4* out);
 void launch_energy(hipStream_t s, int n, const float4* node4, const float4* vel4, int law, double eps, double G,
                    double* partial, double* out);

@@ -42,9 +42,9 @@ constexpr int kBfTileStride = 4 * 17;  // a wave's LDS tile: up to 4 parts of 16
 #define NB_FORCE_WAVES 4
 #endif
 constexpr int kForceWaves = NB_FORCE_WAVES;  // Barnes-Hut force workgroup: waves of 64
-constexpr int kForceVote = 64;
+constexpr int kForceVote = 64;  // bodies per opening decision: one wave, one lane per body
 constexpr int kOrderMax = 32768;  // force workgroups that k_force_order can order (keys in LDS)
-constexpr int kBfFuseMax = 16384;  // single systems up to this size run the integrator inside k_brute  // bodies per opening decision: one wave, one lane per body
+constexpr int kBfFuseMax = 16384;  // single systems up to this size run the integrator inside k_brute
 constexpr int kEnergyLen = 9;   // kinetic, potential, p[3], L[3], mass
 constexpr int kEmpty = -1;
 constexpr int kLocked = -2;

@@ -1107,6 +1107,14 @@ __global__ __launch_bounds__(256) void k_collision(const SysDesc* __restrict__ s
 // ---------------------------------------------------------------------------
 // vertex export (copy_vertices, copy_vertices.cl:3-17): (x, y, z, 1) per body.
 
+// Copies n float4: the fused brute-force step's copy of the moved positions back into
+// node4 after an odd number of steps.  A kernel rather than a hipMemcpyAsync, so that every
+// node of a captured step graph is a kernel node.
+__global__ __launch_bounds__(256) void k_copy4(int n, const float4* __restrict__ src, float4* __restrict__ dst) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 __global__ __launch_bounds__(256) void k_vertices(int n, const float4* __restrict__ node4, float4* __restrict__ out) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
@@ -1269,6 +1277,9 @@ void launch_brute(hipStream_t s, int nblk, int law, bool kahan, int tb, const Sy
 void launch_collision(hipStream_t s, int nblk, bool all, const SysDesc* sd, const SysState* st, const BlkWork* work,
                       const float4* node4, float4* vel4) {
     NB_LAUNCH(k_collision, nblk, 256, s, sd, st, work, node4, vel4, all ? 1 : 0);
+}
+void launch_copy4(hipStream_t s, int n, const float4* src, float4* dst) {
+    NB_LAUNCH(k_copy4, (n + 255) / 256, 256, s, n, src, dst);
 }
 void launch_vertices(hipStream_t s, int n, const float4* node4, float4* out) {
     NB_LAUNCH(k_vertices, (n + 255) / 256, 256, s, n, node4, out);
