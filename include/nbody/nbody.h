@@ -255,6 +255,22 @@ nb_status nb_sim_step_count(nb_sim* sim, int64_t* steps);
 #define NB_ENERGY_LEN 9
 nb_status nb_sim_energy(nb_sim* sim, double* out);
 
+/* nb_sim_energy without allocating: no counterpart in the reference.  For callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops).  Same
+ * kernels, so the same results bit for bit as nb_sim_energy on the same state.  Inputs and
+ * output are caller-owned DEVICE arrays: bodies_xyzm = float4 (x, y, z, m) per body and
+ * vel_xyzw = float4 (vx, vy, vz, unused) per body, both 16-byte aligned; out =
+ * NB_ENERGY_LEN doubles (layout of nb_sim_energy).  law = NB_LAW_PLUMMER (pair potential
+ * -G m_i m_j / sqrt(r^2 + eps^2), eps > 0) or NB_LAW_REFERENCE_UNIT (+G m_i m_j r /
+ * (1 + eps^2)).  Asynchronous on `stream` (NULL: the null stream).  The per-workgroup
+ * partial sums live in a library-owned static device array sized for at most
+ * NB_NOALLOC_MAX_BODIES bodies (4.5 MiB per device, held until the process ends); larger n
+ * is NB_E_INVALID_ARG.  Every call on one device shares that array: calls that overlap on
+ * one device race.  No hipMalloc, hipFree or hipMemset is called. */
+#define NB_NOALLOC_MAX_BODIES (1 << 24)
+nb_status nb_energy_f64_noalloc(const float* bodies_xyzm, const float* vel_xyzw, int32_t n, int32_t law,
+                                double eps, double G, double* out, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Ensembles: many independent systems stepped together.  The reference steps
  * one system per frame (CMyApp::Update, src/MyApp.cpp:550-656); here every

@@ -23,6 +23,7 @@ NB_E_DEPTH_OVERFLOW = 6
 NB_E_NONFINITE = 7
 NB_E_STATE = 8
 NB_E_ADMISSION = 9
+NOALLOC_MAX_BODIES = 1 << 24  # NB_NOALLOC_MAX_BODIES
 
 # enum values of src/SimulationConfig.h:10-38
 LEAPFROG, EULER = 1, 2
@@ -191,6 +192,7 @@ SIGNATURES = {
     "nb_stage_times": (C.c_int, [_P, _FP]),
     "nb_step_async": (C.c_int, [_P, C.c_int32]),
     "nb_sim_energy": (C.c_int, [_P, _DP]),
+    "nb_energy_f64_noalloc": (C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_double, C.c_double, _P, _P]),
     "nb_force_vote_group": (C.c_int32, []),
     "nb_tuning_default": (None, [C.POINTER(NbTuning)]),
     "nb_set_tuning": (C.c_int, [C.POINTER(NbTuning)]),
