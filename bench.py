@@ -328,22 +328,6 @@ def hip_stream():
     return st.value, hip
 
 
-def ensemble_ics(sizes, kinds, seed):
-    from nbodysimulation_amd import ic
-    out = []
-    for i, (n, k) in enumerate(zip(sizes, kinds)):
-        s = seed * 100003 + i
-        if k == 0:
-            p, v, m = ic.plummer(int(n), s)
-        elif k == 1:
-            p, v, m = ic.normal_cloud(int(n), s)
-            m = np.full(int(n), 1.0 / n, np.float32)
-        else:
-            p, v, m = ic.uniform_ball(int(n), s)
-        out.append((p, v, m))
-    return out
-
-
 def bench_ensemble(args):
     """512 independent systems, N log-uniform in [1024, 65536], each a Plummer, Normal(0,1) or
     uniform-ball cloud (equal odds), m = 1/N, eps = 0.01, dt = 1e-3, seed 2; brute force below the
@@ -351,20 +335,19 @@ def bench_ensemble(args):
     step for all systems against the same systems stepped one system per launch."""
     from nbodysimulation_amd import _lib as L
     from nbodysimulation_amd import autotune, device_info
-    from nbodysimulation_amd.engine import NBodyEnsemble, NBodySim, make_config, system_spec
+    from nbodysimulation_amd import workloads as W
+    from nbodysimulation_amd.engine import NBodyEnsemble, NBodySim, make_config
 
     info = device_info()
     if info is None:
         raise SystemExit("no HIP device")
     tune = autotune.load_or_tune(0.5, 0.01, log=log)
     cross = int(tune["crossover_n"])
-    count = 512
-    sizes = autotune.log_uniform_sizes(count, 1024, 65536, seed=2)
-    kinds = np.random.default_rng(2).integers(0, 3, count)
-    ics = ensemble_ics(sizes, kinds, 2)
-    algos = [autotune.choose_algorithm(int(n), cross) for n in sizes]
-    specs = [system_spec(int(n), a, rules=L.RULES_STANDARD, theta=0.5, eps=0.01, dt=1e-3)
-             for n, a in zip(sizes, algos)]
+    count = W.ENSEMBLE_SYSTEMS
+    sizes, kinds = W.ensemble_sizes_kinds()
+    ics = [W.ensemble_ic(i, n, k) for i, (n, k) in enumerate(zip(sizes, kinds))]
+    specs = W.ensemble_specs(sizes, cross)
+    algos = [sp.algorithm for sp in specs]
     steps, warm = args.steps, args.warmup
     log(f"bench: ensemble of {count} systems, {int(sizes.sum())} bodies, crossover N={cross}, "
         f"{sum(a == L.BARNES_HUT for a in algos)} Barnes-Hut / {sum(a != L.BARNES_HUT for a in algos)} brute force")
@@ -533,8 +516,9 @@ def bench_serving(args):
     import threading
 
     from nbodysimulation_amd import _lib as L
-    from nbodysimulation_amd import autotune, device_info, ic
-    from nbodysimulation_amd.engine import NBodyServer, system_spec
+    from nbodysimulation_amd import autotune, device_info
+    from nbodysimulation_amd import workloads as W
+    from nbodysimulation_amd.engine import NBodyServer
 
     info = device_info()
     if info is None:
@@ -542,14 +526,8 @@ def bench_serving(args):
     tune = autotune.load_or_tune(0.5, 0.01, log=log)
     cross = int(tune["crossover_n"])
     R, rate = args.requests, args.rate
-    rng = np.random.default_rng(3)
-    sizes = autotune.log_uniform_sizes(R, 4096, 2097152, seed=3)
-    steps = rng.integers(10, 201, R)
-    thetas = rng.choice([0.3, 0.5, 0.7], R)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, R))
-    arrivals -= arrivals[0]
-    specs = [system_spec(int(n), autotune.choose_algorithm(int(n), cross), rules=L.RULES_STANDARD,
-                         theta=float(t), eps=0.01, dt=1e-3) for n, t in zip(sizes, thetas)]
+    rq = W.serving_requests(R, rate, cross)
+    sizes, steps, arrivals, specs = rq.sizes, rq.steps, rq.arrivals, rq.specs
     work = float((sizes * steps).sum())
     log(f"bench: serving {R} requests at {rate} req/s, {work:.3e} body-steps, budget {args.budget_gib} GiB, "
         f"{args.max_systems} slots, crossover N={cross}")
@@ -559,7 +537,7 @@ def bench_serving(args):
 
     def produce():
         for i in range(R):
-            ready.put(ic.plummer(int(sizes[i]), 3 * 1000003 + i))
+            ready.put(W.serving_ic(i, int(sizes[i])))
 
     threading.Thread(target=produce, daemon=True).start()
 

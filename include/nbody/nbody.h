@@ -214,6 +214,10 @@ nb_status nb_export_force_order(nb_sim* sim, int32_t* order);
 nb_status nb_export_dq(nb_sim* sim, float* dq);
 /* Interactions the last nb_stage_force performed, summed over bodies (synchronises). */
 nb_status nb_force_interactions(nb_sim* sim, int64_t* total);
+/* Interactions of each body in the last force pass, N entries in body index order: Barnes-Hut,
+ * the bodies and cells its vote group accepted for it; brute force, N - 1.  No counterpart in the
+ * reference (it counts nothing); for parity checks on sampled vote groups.  Synchronises. */
+nb_status nb_export_interactions(nb_sim* sim, int32_t* per_body);
 
 /* Per-stage milliseconds of the last step (needs cfg.time_stages). */
 nb_status nb_stage_times(nb_sim* sim, float* ms);
@@ -301,6 +305,8 @@ nb_status nb_ens_upload(nb_ensemble* ens, int32_t system, const float* pos_xyz, 
 nb_status nb_ens_download(nb_ensemble* ens, int32_t system, float* pos_xyz, float* vel_xyz, float* acc_xyz);
 /* nsteps batched steps of every system; synchronises and reports the first failing status. */
 nb_status nb_ens_step(nb_ensemble* ens, int32_t nsteps);
+/* nb_export_interactions for one system of the ensemble. */
+nb_status nb_ens_export_interactions(nb_ensemble* ens, int32_t system, int32_t* per_body);
 /* per-system nb_status after the last step (count entries). */
 nb_status nb_ens_status(nb_ensemble* ens, int32_t* status);
 /* HBM bytes the ensemble holds. */

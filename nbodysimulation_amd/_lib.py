@@ -189,6 +189,8 @@ SIGNATURES = {
     "nb_export_force_order": (C.c_int, [_P, _IP]),
     "nb_export_dq": (C.c_int, [_P, _FP]),
     "nb_force_interactions": (C.c_int, [_P, _I64P]),
+    "nb_export_interactions": (C.c_int, [_P, _IP]),
+    "nb_ens_export_interactions": (C.c_int, [_P, C.c_int32, _IP]),
     "nb_stage_times": (C.c_int, [_P, _FP]),
     "nb_step_async": (C.c_int, [_P, C.c_int32]),
     "nb_sim_energy": (C.c_int, [_P, _DP]),

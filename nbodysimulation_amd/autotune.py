@@ -25,6 +25,7 @@ import numpy as np
 from . import _lib as L
 from . import ic
 from .engine import NBodyEnsemble, device_info, system_spec
+from .workloads import choose_algorithm, log_uniform_sizes  # noqa: F401  (re-exported)
 
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "nbodysimulation_amd", "autotune.json")
 SHAPES = [(tpl, waves) for waves in (4096, 8192, 16384) for tpl in (0, 1, 2)]
@@ -112,10 +113,3 @@ def load_or_tune(theta: float = 0.5, eps: float = 0.01, log=None, retune: bool =
     return res
 
 
-def choose_algorithm(n: int, crossover_n: int) -> int:
-    return L.BARNES_HUT if n >= crossover_n else L.BRUTE_FORCE_GLOBAL
-
-
-def log_uniform_sizes(count: int, lo: int, hi: int, seed: int) -> np.ndarray:
-    r = np.random.default_rng(seed)
-    return np.exp(r.uniform(np.log(lo), np.log(hi), count)).astype(np.int64)

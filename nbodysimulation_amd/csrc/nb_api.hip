@@ -945,6 +945,25 @@ nb_status nb_export_cells(nb_sim* sim, int32_t* level, int32_t* slot, int32_t* c
     return NB_OK;
 }
 
+}  // extern "C"
+
+// Per-body interaction counts of the last force pass: the force kernels store them in acc.w.
+static nb_status interactions_of(Engine& e, int body_off, int n, int32_t* out) {
+    if (n <= 0) return NB_OK;
+    std::vector<float4> h(n);
+    NB_HIPC(hipMemcpyAsync(h.data(), e.acc4 + body_off, sizeof(float4) * n, hipMemcpyDeviceToHost, e.stream));
+    NB_HIPC(hipStreamSynchronize(e.stream));
+    for (int i = 0; i < n; ++i) out[i] = (int32_t)h[i].w;
+    return NB_OK;
+}
+
+extern "C" {
+
+nb_status nb_export_interactions(nb_sim* sim, int32_t* per_body) {
+    if (!sim || (!per_body && sim->n > 0)) return fail(NB_E_INVALID_ARG, "NULL argument");
+    return interactions_of(sim->eng, 0, sim->n, per_body);
+}
+
 nb_status nb_force_interactions(nb_sim* sim, int64_t* total) {
     if (!sim || !total) return fail(NB_E_INVALID_ARG, "NULL argument");
     Engine& e = sim->eng;
@@ -1085,6 +1104,13 @@ nb_status nb_ens_download(nb_ensemble* e, int32_t sys, float* pos, float* vel, f
     return st;
 }
 
+nb_status nb_ens_export_interactions(nb_ensemble* e, int32_t sys, int32_t* per_body) {
+    if (!e || sys < 0 || sys >= (int)e->specs.size()) return fail(NB_E_INVALID_ARG, "bad system index");
+    const nb::SysDesc& d = e->eng.hdesc[sys];
+    if (!per_body && d.n > 0) return fail(NB_E_INVALID_ARG, "NULL argument");
+    return interactions_of(e->eng, d.body_off, d.n, per_body);
+}
+
 nb_status nb_ens_step(nb_ensemble* e, int32_t nsteps) {
     nb::TraceRange tr("nb_ens_step");
     if (!e || nsteps < 0) return fail(NB_E_INVALID_ARG, "bad arguments");
@@ -1100,9 +1126,14 @@ nb_status nb_ens_step(nb_ensemble* e, int32_t nsteps) {
         if (!e->graph) {
             NB_HIPC(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
             ens_enqueue_step(e);
+            const hipError_t le = hipGetLastError();  // a launch that failed to be recorded
             hipGraph_t gr = nullptr;
             hipError_t ce = hipStreamEndCapture(g.stream, &gr);
-            if (ce != hipSuccess) return fail(NB_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+            if (ce == hipSuccess && le != hipSuccess) ce = le;
+            if (ce != hipSuccess) {
+                if (gr) (void)hipGraphDestroy(gr);
+                return fail(NB_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+            }
             ce = hipGraphInstantiate(&e->graph, gr, nullptr, nullptr, 0);
             if (ce != hipSuccess) {
                 (void)hipGraphDestroy(gr);

@@ -195,6 +195,12 @@ class NBodySim:
         self._call("nb_force_interactions", C.byref(t))
         return t.value
 
+    def interactions_per_body(self) -> np.ndarray:
+        """Interactions of each body in the last force pass (body index order)."""
+        out = np.zeros(self.n, np.int32)
+        self._call("nb_export_interactions", out.ctypes.data_as(L._IP))
+        return out
+
     def energy(self) -> dict:
         """FP64 kinetic/potential energy, momentum, angular momentum and mass of the current state (GPU)."""
         out = np.zeros(9, np.float64)
@@ -292,6 +298,13 @@ class NBodyEnsemble:
         mass = np.ascontiguousarray(mass, dtype=np.float32).reshape(-1)
         vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float32).reshape(-1)
         L.check(self.lib.nb_ens_upload(self.h, i, _fp(pos), _fp(vel), _fp(mass)), "nb_ens_upload")
+
+    def interactions_per_body(self, i) -> np.ndarray:
+        """Interactions of each body of system i in the last force pass (body index order)."""
+        out = np.zeros(self.specs[i].num_bodies, np.int32)
+        L.check(self.lib.nb_ens_export_interactions(self.h, i, out.ctypes.data_as(L._IP)),
+                "nb_ens_export_interactions")
+        return out
 
     def download(self, i):
         n = self.specs[i].num_bodies

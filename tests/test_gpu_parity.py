@@ -213,6 +213,57 @@ def test_leapfrog_kdk(orc, gpu):
     assert np.abs(v_g - v1).max() <= 1e-5 * (1 + np.abs(v1).max())
 
 
+def _binary(ecc, eps):
+    """Two bodies of mass 0.5 at separation 1 (apocentre of an orbit of eccentricity ecc),
+    Plummer-softened law, G = 1; also the circular angular velocity."""
+    m = np.array([0.5, 0.5], np.float32)
+    a0 = 1.0 / (1.0 + eps * eps) ** 1.5
+    vrel = np.sqrt(a0) * np.sqrt(1.0 - ecc)
+    p = np.array([[-0.5, 0, 0], [0.5, 0, 0]], np.float32)
+    v = np.array([[0, -0.5 * vrel, 0], [0, 0.5 * vrel, 0]], np.float32)
+    return p, v, m, float(np.sqrt(a0))
+
+
+def _leapfrog_run(algo, p, v, m, dt, nsteps, eps=0.01):
+    extra = dict(rules=L.RULES_STANDARD, theta=0.5) if algo == L.BARNES_HUT else dict(law=L.LAW_PLUMMER)
+    with NBodySim(make_config(2, algo, method=L.LEAPFROG, eps=eps, G=1.0, dt=dt, **extra)) as s:
+        s.upload(p, v, m)
+        s.step(nsteps)
+        p1, v1, _ = s.download()
+        e = s.energy()["total"]
+    return p1, v1, e
+
+
+@pytest.mark.parametrize("algo", [L.BRUTE_FORCE_GLOBAL, L.BARNES_HUT])
+def test_leapfrog_properties_on_gpu(gpu, algo):
+    """The engine's LEAPFROG (brute force: fused closing kick; Barnes-Hut: k_update's opening
+    kick + drift and closing kick) pinned without the oracle: a two-body orbit (Barnes-Hut
+    interacts body with body there, so both paths see the exact pair force) returns to its start
+    after 200 steps forward and 200 with negated velocities, and its energy and phase errors at
+    t = 2 shrink 4x per halving of dt.  Semi-implicit Euler or a drift with the pre-kick velocity
+    would give ratios near 2 and miss the start by 3e-2 or more."""
+    eps, T = 0.01, 2.0
+    p0, v0, m, _ = _binary(0.5, eps)
+    p, v, _ = _leapfrog_run(algo, p0, v0, m, 0.01, 200)
+    assert np.abs(p - p0).max() > 0.1
+    pb, vb, _ = _leapfrog_run(algo, p, -v, m, 0.01, 200)
+    say(f"leapfrog algo={algo}: reversal error {np.abs(pb - p0).max():.2e}")
+    assert np.abs(pb - p0).max() < 2e-5 and np.abs(vb + v0).max() < 2e-5
+    _, _, E0 = _leapfrog_run(algo, p0, v0, m, 0.01, 0)
+    dts = (0.04, 0.02, 0.01)
+    de = [abs(_leapfrog_run(algo, p0, v0, m, dt, int(round(T / dt)))[2] - E0) for dt in dts]
+    pc, vc, mc, w = _binary(0.0, eps)
+    dph = []
+    for dt in dts:
+        pe = _leapfrog_run(algo, pc, vc, mc, dt, int(round(T / dt)))[0].astype(np.float64)
+        sep = pe[1] - pe[0]
+        dph.append(abs((np.arctan2(sep[1], sep[0]) - w * T + np.pi) % (2 * np.pi) - np.pi))
+    say(f"leapfrog algo={algo}: energy errors {de}, phase errors {dph}")
+    for errs in (de, dph):
+        for a, b in zip(errs, errs[1:]):
+            assert 3.3 <= a / b <= 4.7, errs
+
+
 # ---------------------------------------------------------------- failure handling (forced from the host)
 
 def test_tree_overflow_reported(gpu):
