@@ -86,10 +86,12 @@ def parse_args():
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--rules", choices=["standard", "reference"], default="standard")
     ap.add_argument("--theta", type=float, default=0.5)
-    ap.add_argument("--cpu-sample", type=int, default=65536)
+    ap.add_argument("--cpu-sample", type=int, default=32768,
+                    help="bh1m: bodies whose traversal the one-thread CPU figure times")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--no-shares", action="store_true", help="bh1m: skip the per-GPU target-share timing")
+    ap.add_argument("--no-companion", action="store_true", help="bh1m: skip the timed run under the other rule set")
     ap.add_argument("--requests", type=int, default=2000, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=6.0, help="serving: Poisson arrival rate, requests/s")
     ap.add_argument("--budget-gib", type=float, default=96.0, help="serving: HBM budget of the server arena")
@@ -117,11 +119,131 @@ def main():
      "bf4096": bench_bf4096}[args.workload](args)
 
 
-def bench_bh(args):
+REFERENCE_WGS = 256  # the reference's work-group size on an AMD GPU (CL_DEVICE_MAX_WORK_GROUP_SIZE)
 
+
+def run_bh(n, rules, theta, eps, G, dt, steps, warmup, state):
+    """The timed Barnes-Hut workload under one rule set: replayed step graphs for the rate (the
+    production path), the same steps launched stage by stage with HIP events on the simulation's
+    stream for the per-kernel times.  Returns the rate, the stage times and the end state with
+    one more force pass on it."""
+    from nbodysimulation_amd import _lib as L
+    from nbodysimulation_amd.engine import NBodySim, make_config
+
+    pos, vel, mass = state
+    with NBodySim(make_config(n, L.BARNES_HUT, rules=rules, theta=theta, eps=eps, G=G, dt=dt,
+                              time_stages=True)) as timed:
+        timed.upload(pos, vel, mass)
+        if warmup:
+            timed.step(warmup)
+        timed.step(steps)
+        stages = timed.stage_times()  # ms summed over the timed steps
+    sim = NBodySim(make_config(n, L.BARNES_HUT, rules=rules, theta=theta, eps=eps, G=G, dt=dt))
+    sim.upload(pos, vel, mass)
+    if warmup:
+        sim.step(warmup)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step(steps)  # returns after a stream synchronise
+    elapsed = time.perf_counter() - t0
+    out = {"steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps, "elapsed": elapsed,
+           "stages": stages, "exec_inter": sim.interactions(), "tinfo": sim.tree_info()}
+    sim.force_pipeline()
+    t_dl = time.perf_counter()
+    out["end"] = sim.download()
+    out["download_s"] = time.perf_counter() - t_dl
+    out["sim"] = sim
+    return out
+
+
+def engine_force(n, rules, theta, eps, G, pos, vel, mass):
+    """The engine's accelerations on a given state (one force pass) under a rule set."""
+    from nbodysimulation_amd import _lib as L
+    from nbodysimulation_amd.engine import NBodySim, make_config
+    with NBodySim(make_config(n, L.BARNES_HUT, rules=rules, theta=theta, eps=eps, G=G)) as s:
+        s.upload(pos, vel, mass)
+        s.force_pipeline()
+        return s.download()[2]
+
+
+def reference_vote_error(pos, mass, theta, eps, G, engine_acc, groups=8, seed=99):
+    """How far the engine's accelerations are from the reference kernels' own opening rule.
+    calculate_force_local (kernels/calculate_forces.cl:84-145) walks the tree once per
+    work-group and accepts a cell only if work_group_all says it is far enough for every body
+    of the work-group: WGS consecutive bodies in index order (body_idx = get_global_id(0),
+    WGS = CL_DEVICE_MAX_WORK_GROUP_SIZE, src/MyApp.cpp:176-181), under the reference's tree
+    and thresholds (REFERENCE rules).  The oracle emulates exactly that on `groups` sampled
+    work-groups; each engine result (rules -> accelerations) is compared with it and with an
+    FP64 direct sum on the same bodies."""
+    from oracle import oracle as orc
+
+    n = mass.size
+    tree = orc.Tree(pos, mass, 0)  # REFERENCE rules: the reference's tree and thresholds
+    g = np.sort(np.random.default_rng(seed).choice(n // REFERENCE_WGS, groups, replace=False)).astype(np.int32)
+    a_k, _, ni = tree.force(theta, eps, G, vote=REFERENCE_WGS, order=np.arange(n, dtype=np.int32), groups=g,
+                            want_absacc=False)
+    members = (g[:, None] * REFERENCE_WGS + np.arange(REFERENCE_WGS)[None, :]).reshape(-1).astype(np.int32)
+    a64 = orc.direct_f64(pos, mass, orc.LAW_PLUMMER, eps, G, targets=members)
+    n64 = np.linalg.norm(a64, axis=1)
+    nk = np.linalg.norm(a_k[members], axis=1)
+
+    def stats(rel):
+        return {"median": float(np.median(rel)), "p99": float(np.quantile(rel, 0.99))}
+
+    out = {"reference_kernels": {"vote": f"work_group_all over {REFERENCE_WGS} consecutive index-order bodies",
+                                 "rules": "reference", "bodies": int(members.size),
+                                 "interactions_per_body": float(ni[members].mean()),
+                                 "error_vs_fp64": stats(np.linalg.norm(a_k[members] - a64, axis=1) / n64)}}
+    for name, a in engine_acc.items():
+        out[f"engine_{name}"] = {
+            "rel_diff_vs_reference_kernels": stats(np.linalg.norm(a[members] - a_k[members], axis=1) / nk),
+            "error_vs_fp64": stats(np.linalg.norm(a[members] - a64, axis=1) / n64)}
+    return out
+
+
+def cpu_step(pos, vel, mass, rules, theta, eps, G, dt, cap, sample1):
+    """The oracle's Barnes-Hut step timed on this host: bounds, the sequential tree build with
+    summarize, sort and force order (serial code), the per-body traversal of every body (OpenMP
+    over all threads) and the Euler update, each part measured whole.  Also the per-body
+    interaction count I_alg (exact) and a one-thread figure: the serial parts measured again on one
+    thread and the traversal of `sample1` bodies on one thread, scaled to N."""
+    from oracle import oracle as orc
+
+    n = mass.size
+    threads = orc.threads()
+    t = [time.perf_counter()]
+    box, _ = orc.bbox(pos)
+    t.append(time.perf_counter())
+    tree = orc.Tree(pos, mass, rules, cap=cap, box=box)
+    t.append(time.perf_counter())
+    acc, _, ni = tree.force(theta, eps, G, vote=1, want_absacc=False)
+    t.append(time.perf_counter())
+    orc.euler(acc, vel, pos, dt)
+    t.append(time.perf_counter())
+    parts = {"bounds": t[1] - t[0], "build_summarize_sort_serial": t[2] - t[1], "traversal": t[3] - t[2],
+             "update": t[4] - t[3]}
+    full = t[4] - t[0]
+    orc.set_threads(1)
+    try:
+        u = [time.perf_counter()]
+        box1, _ = orc.bbox(pos)
+        tree1 = orc.Tree(pos, mass, rules, cap=cap, box=box1)
+        u.append(time.perf_counter())
+        g = np.sort(np.random.default_rng(7).choice(n, sample1, replace=False)).astype(np.int32)
+        tree1.force(theta, eps, G, vote=1, groups=g, want_absacc=False)
+        u.append(time.perf_counter())
+        orc.euler(acc, vel, pos, dt)
+        u.append(time.perf_counter())
+    finally:
+        orc.set_threads(threads)
+    one = (u[1] - u[0]) + (u[2] - u[1]) * n / sample1 + (u[3] - u[2])
+    return {"full_s": full, "parts_s": parts, "threads": threads, "i_alg": float(ni.sum()),
+            "one_thread_s": one, "one_thread_sample": sample1}
+
+
+def bench_bh(args):
     from nbodysimulation_amd import _lib as L
     from nbodysimulation_amd import device_info, ic
-    from nbodysimulation_amd.engine import NBodySim, make_config
 
     info = device_info()
     if info is None:
@@ -131,82 +253,75 @@ def bench_bh(args):
     log(f"bench: device {info}, N={n}, rules={args.rules}, theta={theta}")
     pos, vel, mass = ic.plummer(n, 1)
 
-    # the same steps twice: replayed graphs (the production path; `value`), then launched stage by
-    # stage with HIP events around every stage on the simulation's stream (per-kernel times)
-    with NBodySim(make_config(n, L.BARNES_HUT, rules=rules, theta=theta, eps=eps, G=G, dt=dt,
-                              time_stages=True)) as timed:
-        timed.upload(pos, vel, mass)
-        if args.warmup:
-            timed.step(args.warmup)
-        timed.step(args.steps)
-        stages = timed.stage_times()  # ms summed over the timed steps
-    sim = NBodySim(make_config(n, L.BARNES_HUT, rules=rules, theta=theta, eps=eps, G=G, dt=dt))
-    sim.upload(pos, vel, mass)
-    if args.warmup:
-        sim.step(args.warmup)
-    sim.sync()
-    t0 = time.perf_counter()
-    sim.step(args.steps)  # returns after a stream synchronise
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    steps_per_s = args.steps / elapsed
-    ms_per_step = 1e3 * elapsed / args.steps
-    exec_inter = sim.interactions()  # last step, executed (wave vote included)
-    tinfo = sim.tree_info()
+    main = run_bh(n, rules, theta, eps, G, dt, args.steps, args.warmup, (pos, vel, mass))
+    steps_per_s, ms_per_step, stages, tinfo = main["steps_per_s"], main["ms_per_step"], main["stages"], main["tinfo"]
+    exec_inter = main["exec_inter"]
     log(f"bench: {steps_per_s:.2f} steps/s, {ms_per_step:.3f} ms/step, stages(ms/step) "
         + ", ".join(f"{k}={v / args.steps:.3f}" for k, v in stages.items() if v))
-
-    # state at the end of the timed region, and one more force pass on it
-    sim.force_pipeline()
-    t_dl = time.perf_counter()
-    p_end, v_end, a_end = sim.download()
-    t_dl = time.perf_counter() - t_dl
+    p_end, v_end, a_end = main["end"]
+    sim = main["sim"]
     # host round trip through the C ABI (pageable buffers): what a caller that hands over
     # host arrays every step would pay on top of the resident-state rate
     t_ul = time.perf_counter()
     sim.upload(p_end, v_end, mass)
     t_ul = time.perf_counter() - t_ul
     sim.close()
-    pcie = {"upload_ms": 1e3 * t_ul, "download_ms": 1e3 * t_dl,
-            "steps_per_s_with_host_round_trip": 1.0 / (elapsed / args.steps + t_ul + t_dl)}
+    pcie = {"upload_ms": 1e3 * t_ul, "download_ms": 1e3 * main["download_s"],
+            "steps_per_s_with_host_round_trip": 1.0 / (main["elapsed"] / args.steps + t_ul + main["download_s"])}
 
     shares = None if args.no_shares else target_shares(p_end, v_end, mass, rules, theta, eps, G, dt)
 
+    # the other rule set on the same workload: its rate, and its accelerations on the same end state
+    other_rules = L.RULES_REFERENCE if rules == L.RULES_STANDARD else L.RULES_STANDARD
+    other_name = "reference" if other_rules == L.RULES_REFERENCE else "standard"
+    companion = None
+    if not args.no_companion:
+        oth = run_bh(n, other_rules, theta, eps, G, dt, args.steps, args.warmup, (pos, vel, mass))
+        oth["sim"].close()
+        companion = {"rules": other_name, "value": oth["steps_per_s"], "unit": "timesteps/s",
+                     "ms_per_step": oth["ms_per_step"],
+                     "stage_ms_per_step": {k: v / args.steps for k, v in oth["stages"].items()},
+                     "executed_interactions_per_step": oth["exec_inter"]}
+        log(f"bench: {other_name} rules {oth['steps_per_s']:.2f} steps/s")
+
     from oracle import oracle as orc
 
-    out_err = None
+    out_err = ref_vote = None
     if not args.no_error:
         tg = np.random.default_rng(12345).choice(n, 1024, replace=False).astype(np.int32)
         a64 = orc.direct_f64(p_end, mass, orc.LAW_PLUMMER, eps, G, targets=tg)
-        rel = np.linalg.norm(a_end[tg] - a64, axis=1) / np.linalg.norm(a64, axis=1)
+        n64 = np.linalg.norm(a64, axis=1)
+        rel = np.linalg.norm(a_end[tg] - a64, axis=1) / n64
         out_err = {"median": float(np.median(rel)), "p99": float(np.quantile(rel, 0.99)), "targets": 1024,
-                   "against": "FP64 direct sum"}
+                   "against": "FP64 direct sum", "state": "end of the timed steps"}
         log(f"bench: force rel err vs FP64 median {out_err['median']:.3e} p99 {out_err['p99']:.3e}")
+        a_other = engine_force(n, other_rules, theta, eps, G, p_end, v_end, mass)
+        rel_o = np.linalg.norm(a_other[tg] - a64, axis=1) / n64
+        if companion is not None:
+            companion["force_error"] = {"median": float(np.median(rel_o)), "p99": float(np.quantile(rel_o, 0.99)),
+                                        "targets": 1024, "against": "FP64 direct sum",
+                                        "state": "the same end state as force_error"}
+        ref_vote = reference_vote_error(p_end, mass, theta, eps, G, {args.rules: a_end, other_name: a_other})
+        log("bench: vs the reference kernels' work-group vote: "
+            + ", ".join(f"{k} median {v['rel_diff_vs_reference_kernels']['median']:.3e}"
+                        for k, v in ref_vote.items() if k.startswith("engine_")))
 
-    # oracle on the same state: per-body interaction count (I_alg) and the CPU baseline
+    # the oracle's step on the same state: the CPU baseline, and the per-body interaction count I_alg
+    c = cpu_step(p_end, v_end, mass, rules, theta, eps, G, dt, tinfo["capacity"], min(args.cpu_sample, n))
+    i_alg = c["i_alg"]
     cpu = None
-    t_a = time.perf_counter()
-    box, _ = orc.bbox(p_end)
-    t_b = time.perf_counter()
-    tree = orc.Tree(p_end, mass, rules, cap=tinfo["capacity"], box=box)
-    t_c = time.perf_counter()
-    sample = min(args.cpu_sample, n)
-    groups = np.sort(np.random.default_rng(7).choice(n, sample, replace=False)).astype(np.int32)
-    _, _, ni = tree.force(theta, eps, G, vote=1, groups=groups)
-    t_d = time.perf_counter()
-    members = tree.force_order[groups]
-    i_alg = float(ni[members].mean()) * n
-    acc_dummy = np.zeros((n, 3), np.float32)
-    t_e = time.perf_counter()
-    orc.euler(acc_dummy, np.zeros((n, 3), np.float32), p_end, dt)
-    t_f = time.perf_counter()
-    cpu_step_s = (t_b - t_a) + (t_c - t_b) + (t_d - t_c) * n / sample + (t_f - t_e)
     if not args.no_cpu_baseline:
-        cpu = {"value": 1.0 / cpu_step_s, "unit": "timesteps/s", "cores": orc.threads(), "host": host_cpu(), "kind": "port",
-               "sample": (f"one step of the same workload on the oracle: bounds, sequential tree build, summarize "
-                          f"and sort over all {n} bodies, per-body traversal of {sample} sampled bodies scaled "
-                          f"to {n}, Euler update; {cpu_step_s:.2f} s per step")}
-        log(f"bench: cpu baseline {cpu['value']:.4f} steps/s ({cpu_step_s:.2f} s/step, {cpu['cores']} threads)")
+        cpu = {"value": 1.0 / c["full_s"], "unit": "timesteps/s", "cores": c["threads"], "host": host_cpu(),
+               "kind": "port",
+               "sample": (f"one whole step of the same workload on the oracle, measured: bounds, the sequential "
+                          f"(one-thread) tree build, summarize, sort and force order, the per-body traversal of "
+                          f"all {n} bodies on {c['threads']} threads, Euler update; {c['full_s']:.2f} s"),
+               "parts_s": c["parts_s"],
+               "one_core": {"value": 1.0 / c["one_thread_s"], "unit": "timesteps/s", "cores": 1,
+                            "sample": (f"serial parts measured on one thread; traversal of {c['one_thread_sample']} "
+                                       f"sampled bodies on one thread scaled to {n}; {c['one_thread_s']:.1f} s per step")}}
+        log(f"bench: cpu baseline {cpu['value']:.4f} steps/s ({c['full_s']:.2f} s/step, {c['threads']} threads); "
+            f"one thread {1.0 / c['one_thread_s']:.4f} steps/s")
 
     force_s = stages["force"] * 1e-3 / args.steps
     achieved = FLOPS_PER_INTERACTION * i_alg / force_s / 1e12
@@ -242,6 +357,8 @@ def bench_bh(args):
         "host_round_trip": pcie,
         "target_shares": shares,
         "force_error": out_err,
+        "other_rules": companion,
+        "reference_vote_error": ref_vote,
         "roofline": {"bound": "mfma", "note": "FP32 vector ALU (no MFMA); peak = MI355X FP32 157.3 TFLOP/s",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
@@ -253,11 +370,14 @@ def bench_bh(args):
 
 
 # Algorithmic HBM bytes per step of the tree stages and the update (N bodies, C cells used):
-# bounds 12N, reset 32(C+1), build 16N+20C, summarize 16N+72C, sort 24N+64C (positions in,
-# two permutations out; per cell the child slots and the 8 per-slot counts), update 60N.
+# bounds 12N, reset 32(C+1), build 16N+20C, summarize 16N+72C, sort 4N+40C (one permutation out;
+# per cell its child slots and a count), update 60N.  This design's sort reads each body's position
+# and writes two permutations (tree order and force order) and reads 8 per-slot counts per cell:
+# 24N+64C, reported beside the credited figure.
 STAGE_BYTES = {"bounds": lambda n, c: 12 * n, "reset": lambda n, c: 32 * (c + 1),
                "build": lambda n, c: 16 * n + 20 * c, "summarize": lambda n, c: 16 * n + 72 * c,
-               "sort": lambda n, c: 24 * n + 64 * c, "update": lambda n, c: 60 * n}
+               "sort": lambda n, c: 4 * n + 40 * c, "update": lambda n, c: 60 * n}
+SORT_BYTES_THIS_DESIGN = lambda n, c: 24 * n + 64 * c  # noqa: E731
 PEAK_HBM_GBS = 8000.0
 
 
@@ -269,6 +389,10 @@ def stage_bandwidth(n, cells, stages, steps):
         if ms > 0:
             gbs = f(n, cells) / (ms * 1e-3) / 1e9
             out[k] = {"algorithmic_bytes": f(n, cells), "ms": ms, "GB_s": gbs, "frac": gbs / PEAK_HBM_GBS}
+    if "sort" in out:
+        b = SORT_BYTES_THIS_DESIGN(n, cells)
+        out["sort"]["bytes_this_design"] = b
+        out["sort"]["GB_s_this_design"] = b / (out["sort"]["ms"] * 1e-3) / 1e9
     return out
 
 
@@ -326,6 +450,41 @@ def hip_stream():
     if hip.hipStreamCreate(C.byref(st)) != 0:
         raise RuntimeError("hipStreamCreate failed")
     return st.value, hip
+
+
+def ensemble_force_error(specs, ics, systems=32, targets=128, seed=5):
+    """FP64 force error of the ensemble workload: one force pass on the initial state of `systems`
+    sampled systems (each system's arithmetic is the same alone and batched), against an FP64
+    direct sum on `targets` sampled bodies of each."""
+    from nbodysimulation_amd import _lib as L
+    from nbodysimulation_amd.engine import NBodySim, make_config
+    from oracle import oracle as orc
+
+    r = np.random.default_rng(seed)
+    rel = {L.BARNES_HUT: [], L.BRUTE_FORCE_GLOBAL: []}
+    for i in np.sort(r.choice(len(specs), systems, replace=False)):
+        sp, (p, v, m) = specs[i], ics[i]
+        n = sp.num_bodies
+        with NBodySim(make_config(n, sp.algorithm, rules=sp.rules, law=L.LAW_PLUMMER, theta=sp.theta,
+                                  eps=sp.epsilon, G=sp.gravitational_constant,
+                                  cell_capacity=2 * n + 1024)) as s:
+            s.upload(p, v, m)
+            if sp.algorithm == L.BARNES_HUT:
+                s.force_pipeline()
+            else:
+                s.brute_force()
+            a = s.download()[2]
+        tg = np.sort(r.choice(n, min(targets, n), replace=False)).astype(np.int32)
+        a64 = orc.direct_f64(p, m, orc.LAW_PLUMMER, sp.epsilon, sp.gravitational_constant, targets=tg)
+        rel[sp.algorithm].append(np.linalg.norm(a[tg] - a64, axis=1) / np.linalg.norm(a64, axis=1))
+    allr = np.concatenate(rel[L.BARNES_HUT] + rel[L.BRUTE_FORCE_GLOBAL])
+    out = {"median": float(np.median(allr)), "p99": float(np.quantile(allr, 0.99)), "systems": systems,
+           "targets_per_system": targets, "against": "FP64 direct sum"}
+    for name, k in (("barnes_hut", L.BARNES_HUT), ("brute_force", L.BRUTE_FORCE_GLOBAL)):
+        if rel[k]:
+            x = np.concatenate(rel[k])
+            out[name] = {"median": float(np.median(x)), "p99": float(np.quantile(x, 0.99)), "systems": len(rel[k])}
+    return out
 
 
 def bench_ensemble(args):
@@ -391,6 +550,8 @@ def bench_ensemble(args):
 
     t_plain_shared = plain(True)
     t_plain_own = plain(False)
+    err = ensemble_force_error(specs, ics)
+    log(f"bench: ensemble force error vs FP64 median {err['median']:.3e} p99 {err['p99']:.3e}")
     bodies = int(sizes.sum())
     line = {
         "metric": "ensemble steps/s (512 systems, one batched graph-captured step)",
@@ -407,6 +568,7 @@ def bench_ensemble(args):
             "speedup_vs_shared_stream": t_plain_shared / t_batched,
             "speedup_vs_per_system_streams": t_plain_own / t_batched},
         "autotune": tune,
+        "force_error": err,
     }
     log(f"bench: batched {steps / t_batched:.2f} ensemble steps/s; plain shared stream {steps / t_plain_shared:.2f}, "
         f"own streams {steps / t_plain_own:.2f}")
@@ -641,9 +803,18 @@ def bench_bf4096(args):
             s.step(max(args.warmup, 1))
             s.step(steps)
             st = s.stage_times()
+        # force error of one pass on the initial state against an FP64 direct sum of the same law
+        with NBodySim(make_config(n, L.BRUTE_FORCE_GLOBAL, **cfg)) as s:
+            s.upload(pos, vel, mass)
+            s.brute_force()
+            a32 = s.download()[2]
+        a64 = orc.direct_f64(pos, mass, law, 0.01, 1.0)
+        rel = np.linalg.norm(a32 - a64, axis=1) / np.linalg.norm(a64, axis=1)
         res[name] = {"steps_per_s": steps / wall, "brute_kernel_ms": st["brute"] / steps,
                      "update_kernel_ms": st["update"] / steps,
-                     "steps_per_s_repeats": [steps / w for w in walls]}
+                     "steps_per_s_repeats": [steps / w for w in walls],
+                     "force_error": {"median": float(np.median(rel)), "p99": float(np.quantile(rel, 0.99)),
+                                     "targets": n, "against": "FP64 direct sum of the same law"}}
         log(f"bench: bf4096 {name}: {steps / wall:.1f} steps/s, brute {st['brute'] / steps * 1e3:.2f} us")
     cpu = None
     if not args.no_cpu_baseline:

@@ -38,6 +38,15 @@ int orc_threads(void) {
 #endif
 }
 
+/* Threads of the parallel loops that follow (the CPU baseline's one-thread figure). */
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 orc_tree* orc_tree_new(int n, int cap) {
     orc_tree* t = (orc_tree*)calloc(1, sizeof(orc_tree));
     t->n = n;
@@ -484,7 +493,7 @@ void orc_force_bh(const orc_tree* t, const float* pos, const float* mass, const 
                             a[1] = fmaf(dy, F, a[1]);
                             a[2] = fmaf(dz, F, a[2]);
                         }
-                        aa[i] += fabs((double)F) * sqrt((double)dx * dx + (double)dy * dy + (double)dz * dz);
+                        if (absacc) aa[i] += fabs((double)F) * sqrt((double)dx * dx + (double)dy * dy + (double)dz * dz);
                         ni[i] += 1;
                     }
                 } else {

@@ -83,6 +83,7 @@ void orc_energy_momentum_f64(int law, double eps, double G, int n, const float* 
 int orc_bh_step(int rules, int nsteps, int n, int cap, float theta, float eps, float G, float dt,
                 int vote, int kahan, float* pos, float* vel, const float* mass, float* acc);
 int orc_threads(void);
+void orc_set_threads(int n);
 
 #ifdef __cplusplus
 }

@@ -56,6 +56,7 @@ _SIG = {
     "orc_bh_step": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
                               C.c_int, C.c_int, _FP, _FP, _FP, _FP]),
     "orc_threads": (C.c_int, []),
+    "orc_set_threads": (None, [C.c_int]),
 }
 
 _lib = None
@@ -162,19 +163,21 @@ class Tree:
     def dq(self, theta, eps):
         return dq_table(self.rules, self.box, theta, eps)
 
-    def force(self, theta, eps, G, vote=64, kahan=False, groups=None, order=None):
-        """Barnes-Hut accelerations (n,3); also sum |term| per body and interactions per body.
-        Vote groups are `vote` consecutive bodies of `order` (default: the force order, as the
-        kernel forms them); `groups` selects group indices of that order."""
+    def force(self, theta, eps, G, vote=64, kahan=False, groups=None, order=None, want_absacc=True):
+        """Barnes-Hut accelerations (n,3); also sum |term| per body (None unless want_absacc)
+        and interactions per body.  Vote groups are `vote` consecutive bodies of `order`
+        (default: the force order, as the kernel forms them); `groups` selects group indices
+        of that order."""
         dq = self.dq(theta, eps)
         order = self.force_order if order is None else np.ascontiguousarray(order, dtype=np.int32)
         acc = np.zeros(3 * self.n, np.float32)
-        absacc = np.zeros(self.n, np.float64)
+        absacc = np.zeros(self.n, np.float64) if want_absacc else None
         ninter = np.zeros(self.n, np.int64)
         g = None if groups is None else np.ascontiguousarray(groups, dtype=np.int32)
         self.lib.orc_force_bh(self.h, fp(self.pos), fp(self.mass), ip(order), fp(dq), eps, G, vote,
                               int(kahan), ip(g), 0 if g is None else g.size, fp(acc),
-                              absacc.ctypes.data_as(_DP), ninter.ctypes.data_as(_I64P))
+                              None if absacc is None else absacc.ctypes.data_as(_DP),
+                              ninter.ctypes.data_as(_I64P))
         return acc.reshape(-1, 3), absacc, ninter
 
 
@@ -255,3 +258,7 @@ def bh_steps(pos, vel, mass, rules, nsteps, theta, eps, G, dt, vote=64, kahan=Fa
 
 def threads():
     return load().orc_threads()
+
+
+def set_threads(n):
+    load().orc_set_threads(int(n))
