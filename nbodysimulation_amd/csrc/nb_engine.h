@@ -25,7 +25,8 @@ void launch_tree_clear(hipStream_t s, int nblk, const SysDesc* sd, const SysStat
                        int* child, int* cpend);
 void launch_tree_init(hipStream_t s, int nsys, const int* slots, SysState* st);  // slots NULL: 0..nsys-1
 void launch_build(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
-                  const float4* node4, int* child, int* cparent, int* cpend, int stride, int exclude);
+                  const float4* node4, int* child, int* cparent, int* cpend, float4* cdata, int stride,
+                  int exclude);
 void launch_summarize(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
                       float4* node4, const int* child, const int* cparent, int* cpend, int* ccount, int* slotoff,
                       float4* cdata);

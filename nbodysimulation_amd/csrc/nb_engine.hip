@@ -339,7 +339,7 @@ void Engine::build() {
     TraceRange tr("nb:build");
     for (int ph = 0; ph < kBuildPhases; ++ph)
         launch_build(stream, tab_len[T_PHASE0 + ph], d_desc, d_state, tab[T_PHASE0 + ph], node4, child, cparent,
-                     cpend, kPhaseStride[ph], ph == 0 ? 0 : kPhaseStride[ph - 1]);
+                     cpend, cdata, kPhaseStride[ph], ph == 0 ? 0 : kPhaseStride[ph - 1]);
 }
 void Engine::summarize() {
     TraceRange tr("nb:summarize");

@@ -29,108 +29,10 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include "nb_device.h"
 #include "nb_internal.h"
 
 namespace nb {
-
-// ---------------------------------------------------------------------------
-// helpers
-
-__device__ __forceinline__ int ld_c(const int* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ float ld_cf(const float* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_c(int* p, int v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_cf(float* p, float v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_rel(int* p, int v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ bool cas_c(int* p, int expect, int desired) {
-    return __hip_atomic_compare_exchange_strong(p, &expect, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// DPP row reduction (16 lanes) followed by a read of the four row results:
-// every lane returns the wave-wide min / max.
-template <int CTRL>
-__device__ __forceinline__ float dppc(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
-}
-__device__ __forceinline__ float wave_min(float v) {
-    v = fminf(v, dppc<0xB1>(v));   // quad_perm [1,0,3,2]
-    v = fminf(v, dppc<0x4E>(v));   // quad_perm [2,3,0,1]
-    v = fminf(v, dppc<0x124>(v));  // row_ror:4
-    v = fminf(v, dppc<0x128>(v));  // row_ror:8
-    float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
-    float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
-    float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
-    float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
-    return fminf(fminf(r0, r1), fminf(r2, r3));
-}
-__device__ __forceinline__ float wave_max(float v) {
-    v = fmaxf(v, dppc<0xB1>(v));
-    v = fmaxf(v, dppc<0x4E>(v));
-    v = fmaxf(v, dppc<0x124>(v));
-    v = fmaxf(v, dppc<0x128>(v));
-    float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
-    float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
-    float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
-    float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
-    return fmaxf(fmaxf(r0, r1), fmaxf(r2, r3));
-}
-__device__ __forceinline__ int wave_max_i(int v) {
-    for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-// Root cell box.  REFERENCE: the bounding box (build_tree.cl:53-54).  STANDARD:
-// cube of half-edge r = half the largest extent about the box centre.
-__device__ __forceinline__ void root_box(int rules, const float* box, float& lx, float& ly, float& lz,
-                                         float& hx, float& hy, float& hz) {
-    if (rules == 0) {
-        lx = box[0]; ly = box[1]; lz = box[2]; hx = box[3]; hy = box[4]; hz = box[5];
-        return;
-    }
-    float r = 0.5f * fmaxf(fmaxf(box[3] - box[0], box[4] - box[1]), box[5] - box[2]);
-    float cx = (box[0] + box[3]) * 0.5f, cy = (box[1] + box[4]) * 0.5f, cz = (box[2] + box[5]) * 0.5f;
-    lx = cx - r; hx = cx + r;
-    ly = cy - r; hy = cy + r;
-    lz = cz - r; hz = cz + r;
-}
-
-// Octant 4*[y<=cy] + 2*[x<=cx] + [z<=cz] (calculateOctantIdx, build_tree.hcl:14-29,
-// ties to the lower half) and the halving that follows (adjustBoundaryValues,
-// build_tree.hcl:31-54; REFERENCE rules halve y by the x comparison, :42).
-// Under REFERENCE rules level 0 is the root's single hop into slot 0.
-__device__ __forceinline__ int level_digit(int rules, int lvl, float px, float py, float pz, float& lx,
-                                           float& ly, float& lz, float& hx, float& hy, float& hz) {
-    if (rules == 0 && lvl == 0) return 0;
-    float cx = (lx + hx) * 0.5f, cy = (ly + hy) * 0.5f, cz = (lz + hz) * 0.5f;
-    bool gx = px > cx, gy = py > cy, gz = pz > cz;
-    int d = (gy ? 0 : 4) + (gx ? 0 : 2) + (gz ? 0 : 1);
-    if (gx) lx = cx; else hx = cx;
-    bool ys = rules == 0 ? gx : gy;
-    if (ys) ly = cy; else hy = cy;
-    if (gz) lz = cz; else hz = cz;
-    return d;
-}
-
-// Order LDS accesses of one wave (same-wave LDS operations complete in order).
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ __forceinline__ bool finite3(float4 p) {
-    return isfinite(p.x) && isfinite(p.y) && isfinite(p.z);
-}
 
 // ---------------------------------------------------------------------------
 // bounds: exact min/max over all bodies of each system, canonical +0, plus the
@@ -298,7 +200,7 @@ template <int BLK>  // workgroup size: build_block(stride)
 __global__ __launch_bounds__(BLK) void k_build(const SysDesc* __restrict__ sd, SysState* st,
                                                const BlkWork* __restrict__ work,
                                                const float4* __restrict__ node4, int* child, int* cparent,
-                                               int* cpend, int stride, int exclude) {
+                                               int* cpend, float4* cdata, int stride, int exclude) {
     const BlkWork w = work[blockIdx.x];
     const SysDesc d = sd[w.sys];
     SysState* S = st + w.sys;
@@ -315,9 +217,10 @@ __global__ __launch_bounds__(BLK) void k_build(const SysDesc* __restrict__ sd, S
     int* C = child + 8 * (size_t)d.cell_off;
     int* par = cparent + d.cell_off;
     int* pend = cpend + d.cell_off;
+    float4* CD = cdata + 8 * (size_t)d.cell_off;
     float rlx, rly, rlz, rhx, rhy, rhz;
     root_box(d.rules, S->box, rlx, rly, rlz, rhx, rhy, rhz);
-    float px = 0.f, py = 0.f, pz = 0.f;
+    float px = 0.f, py = 0.f, pz = 0.f, pm = 0.f;
     float lx = rlx, ly = rly, lz = rlz, hx = rhx, hy = rhy, hz = rhz;
     int b = 0, node = 0, lvl = 0, dg = 0;
     int depth = 1;
@@ -327,7 +230,7 @@ __global__ __launch_bounds__(BLK) void k_build(const SysDesc* __restrict__ sd, S
     auto start = [&](int body) {
         b = body;
         const float4 p = node4[d.node_off + b];
-        px = p.x; py = p.y; pz = p.z;
+        px = p.x; py = p.y; pz = p.z; pm = p.w;
         lx = rlx; ly = rly; lz = rlz; hx = rhx; hy = rhy; hz = rhz;
         node = 0;
         lvl = 0;
@@ -385,6 +288,7 @@ __global__ __launch_bounds__(BLK) void k_build(const SysDesc* __restrict__ sd, S
                 fresh = true;
                 if (v == kEmpty) {
                     if (cas_c(C + slot, kEmpty, b)) {
+                        CD[slot] = make_float4(px, py, pz, pm);  // the body's entry of the cell's child block
                         depth = max(depth, lvl);
                         active = false;
                     }
@@ -446,27 +350,33 @@ __global__ __launch_bounds__(BLK) void k_build(const SysDesc* __restrict__ sd, S
             } else {
                 float ax = lx, ay = ly, az = lz, bx = hx, by = hy, bz = hz;
                 float qx0 = lx, qy0 = ly, qz0 = lz, qx1 = hx, qy1 = hy, qz1 = hz;
-                int parent = node;
+                int parent = node, pslot = dg, lb = 0, lq = 0;
                 for (int i = 0; i < k; ++i) {
                     const int c = base + i;
                     const int l2 = lvl + 1 + i;
                     const int db2 = level_digit(d.rules, l2, px, py, pz, ax, ay, az, bx, by, bz);
                     const int dq2 = level_digit(d.rules, l2, q4.x, q4.y, q4.z, qx0, qy0, qz0, qx1, qy1, qz1);
-                    par[c] = parent;
+                    par[c] = (parent << 3) | pslot;  // parent cell and the slot holding c
                     st_c(pend + c, (i < k - 1) ? 1 : 0);
                     if (i < k - 1) {
                         st_c(C + 8 * c + db2, n + c + 1);
                     } else {
                         st_c(C + 8 * c + db2, b);
                         st_c(C + 8 * c + dq2, v);
+                        lb = 8 * c + db2;
+                        lq = 8 * c + dq2;
                     }
                     parent = c;
+                    pslot = db2;
                 }
                 atomicAdd(pend + node, 1);  // the slot now holds a cell instead of a body
                 // chain stores are complete (coherent) before the chain becomes reachable
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
                 __builtin_amdgcn_s_waitcnt(0);
                 st_c(C + slot, n + base);
+                // the two bodies' entries of the last cell's child block (read by summarize, a later kernel)
+                CD[lb] = make_float4(px, py, pz, pm);
+                CD[lq] = q4;
                 depth = max(depth, lvl + k);
             }
             active = false;
@@ -486,14 +396,18 @@ __global__ __launch_bounds__(BLK) void k_build(const SysDesc* __restrict__ sd, S
 
 // ---------------------------------------------------------------------------
 // summarize: bottom-up mass, centre of mass and body count (saturate_tree.cl:6-146)
-// without spin-waits.  Each cell also gets its children's (x, y, z, m) in one
-// contiguous 128-byte block (cdata), so the force traversal fetches a cell's
-// slots and all its children's data in a single round trip.  A lane starts at a cell whose children are all bodies;
-// after finishing a cell it decrements the parent's count of unfinished child
-// cells, and the lane that brings it to zero goes on with the parent.  Children
-// are summed in slot order (deterministic; the reference's late children are
-// summed LIFO, saturate_tree.cl:96-120): mass += m_k, com = fmaf(x_k, m_k, com),
-// then com * (1/mass).
+// without spin-waits.  Each cell ends with its children's (x, y, z, m) in one
+// contiguous 128-byte block (cdata), so the force traversal fetches a cell's slots
+// and all its children's data in a single round trip, and the inclusive per-slot
+// body counts (slotoff) for the sort.  The build already wrote every body's entry of
+// its cell's block; a finished cell writes its own (COM, mass) and body count into its
+// parent's blocks, so a cell reads three contiguous blocks (slots, child data, counts)
+// in one batch of loads instead of gathering its children.  A lane starts at a cell whose
+// children are all bodies; after finishing a cell it decrements the parent's count of
+// unfinished child cells, and the lane that brings it to zero goes on with the parent.
+// Children are summed in slot order (deterministic; the reference's late children are
+// summed LIFO, saturate_tree.cl:96-120): mass += m_k, com = fmaf(x_k, m_k, com), then
+// com * (1/mass).
 
 __global__ __launch_bounds__(256) void k_summarize(const SysDesc* __restrict__ sd, const SysState* __restrict__ st,
                                                    const BlkWork* __restrict__ work, float4* node4,
@@ -514,13 +428,39 @@ __global__ __launch_bounds__(256) void k_summarize(const SysDesc* __restrict__ s
         int mx = max(max(max(a.x, a.y), max(a.z, a.w)), max(max(bq.x, bq.y), max(bq.z, bq.w)));
         if (mx >= n) return;  // has child cells: its last finished child will handle it
     }
-    float4* nodes = node4 + d.node_off;
-    int* cnt_base = ccount + d.cell_off;
+    float4* CD = cdata + 8 * (size_t)d.cell_off;
+    int* SO = slotoff + 8 * (size_t)d.cell_off;
     int c = ci;
     for (;;) {
-        const int4* c4 = reinterpret_cast<const int4*>(C + 8 * c);
-        int4 a = c4[0], bq = c4[1];
-        int vs[8] = {a.x, a.y, a.z, a.w, bq.x, bq.y, bq.z, bq.w};
+        // the cell's slots, child block and per-slot counts in one batch of loads (coherent:
+        // the entries of child cells come from other lanes of this kernel)
+        int vs[8];
+        float4 q[8];
+        int sc[8];
+        {
+            const unsigned long long* s8 = reinterpret_cast<const unsigned long long*>(C + 8 * c);
+            const unsigned long long* d8 = reinterpret_cast<const unsigned long long*>(CD + 8 * c);
+            const unsigned long long* o8 = reinterpret_cast<const unsigned long long*>(SO + 8 * c);
+            unsigned long long sv[4], dv[16], ov[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sv[k] = ld_c64(s8 + k);
+#pragma unroll
+            for (int k = 0; k < 16; ++k) dv[k] = ld_c64(d8 + k);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) ov[k] = ld_c64(o8 + k);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                vs[2 * k] = (int)(unsigned)sv[k];
+                vs[2 * k + 1] = (int)(unsigned)(sv[k] >> 32);
+                sc[2 * k] = (int)(unsigned)ov[k];
+                sc[2 * k + 1] = (int)(unsigned)(ov[k] >> 32);
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                q[k] = make_float4(__uint_as_float((unsigned)dv[2 * k]), __uint_as_float((unsigned)(dv[2 * k] >> 32)),
+                                   __uint_as_float((unsigned)dv[2 * k + 1]),
+                                   __uint_as_float((unsigned)(dv[2 * k + 1] >> 32)));
+        }
         float m = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
         int cnt = 0;
         int incl[8];  // bodies in slots 0..k
@@ -528,44 +468,42 @@ __global__ __launch_bounds__(256) void k_summarize(const SysDesc* __restrict__ s
         for (int k = 0; k < 8; ++k) {
             const int v = vs[k];
             if (v >= 0) {
-                float x, y, z, wm;
-                int sub;
-                if (v < n) {
-                    float4 t = nodes[v];
-                    x = t.x; y = t.y; z = t.z; wm = t.w;
-                    sub = 1;
-                } else {
-                    const float* f = reinterpret_cast<const float*>(nodes + v);
-                    x = ld_cf(f); y = ld_cf(f + 1); z = ld_cf(f + 2); wm = ld_cf(f + 3);
-                    sub = ld_c(cnt_base + (v - n));
-                }
-                cnt += sub;
+                const float wm = q[k].w;
+                cnt += v < n ? 1 : sc[k];
                 m = m + wm;
-                cx = fmaf(x, wm, cx);
-                cy = fmaf(y, wm, cy);
-                cz = fmaf(z, wm, cz);
-                cdata[8 * ((size_t)d.cell_off + c) + k] = make_float4(x, y, z, wm);
+                cx = fmaf(q[k].x, wm, cx);
+                cy = fmaf(q[k].y, wm, cy);
+                cz = fmaf(q[k].z, wm, cz);
             }
             incl[k] = cnt;
         }
-        int4* so = reinterpret_cast<int4*>(slotoff + 8 * ((size_t)d.cell_off + c));
+        const float inv = 1.0f / m;
+        const float4 com = make_float4(cx * inv, cy * inv, cz * inv, m);
+        int4* so = reinterpret_cast<int4*>(SO + 8 * c);
+        bool more = false;
+        int p = 0;
+        if (c != 0) {
+            const int pk = cparent[d.cell_off + c];
+            p = pk >> 3;
+            const int ps = pk & 7;
+            float* f = reinterpret_cast<float*>(CD + 8 * p + ps);
+            st_cf(f, com.x);
+            st_cf(f + 1, com.y);
+            st_cf(f + 2, com.z);
+            st_cf(f + 3, com.w);
+            st_c(SO + 8 * p + ps, cnt);
+            // the coherent stores into the parent's blocks complete before its counter
+            // drops, so the lane that takes the counter to zero reads them (sc1 loads)
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_s_waitcnt(0);
+            more = __hip_atomic_fetch_add(cpend + d.cell_off + p, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 1;
+        }
+        // the cell's own outputs: read by later kernels only
         so[0] = make_int4(incl[0], incl[1], incl[2], incl[3]);
         so[1] = make_int4(incl[4], incl[5], incl[6], incl[7]);
-        const float inv = 1.0f / m;
-        float* o = reinterpret_cast<float*>(nodes + n + c);
-        st_cf(o, cx * inv);
-        st_cf(o + 1, cy * inv);
-        st_cf(o + 2, cz * inv);
-        st_cf(o + 3, m);
-        st_c(cnt_base + c, cnt);
-        if (c == 0) break;
-        const int p = cparent[d.cell_off + c];
-        // this cell's coherent stores complete before the parent's counter drops,
-        // so the lane that takes the counter to zero reads them (sc1 loads)
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_s_waitcnt(0);
-        const int old = __hip_atomic_fetch_add(cpend + d.cell_off + p, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (old != 1) break;
+        node4[d.node_off + n + c] = com;
+        ccount[d.cell_off + c] = cnt;
+        if (!more) break;
         c = p;
     }
 }
@@ -577,23 +515,6 @@ __global__ __launch_bounds__(256) void k_summarize(const SysDesc* __restrict__ s
 // cell on the path; summarize stored per-slot counts.  One lane per body
 // re-derives its path (the same octant digits as the build) and adds them up:
 // no waiting, no dependence on the order in which cells were allocated.
-
-// Force order (the vote groups' order): depth-first with each cell's children
-// visited along a 3-D Hilbert curve, so consecutive bodies stay in face-sharing
-// cells.  24 orientations, the root's is 0.  kHPos[s] packs, 3 bits per octant,
-// the octant's place in the visiting order of orientation s; kHNext[s] packs,
-// 5 bits per octant, the child's orientation.  The tables follow the Hilbert
-// index of Skilling ("Programming the Hilbert curve", AIP Conf. Proc. 707, 2004)
-// on the octree's octants 4[y<=c]+2[x<=c]+[z<=c].
-__constant__ unsigned kHPos[24] = {
-    0x07e6a5, 0x1ce715, 0xc78aa3, 0x97e681, 0x8f8ab1, 0x05afa5, 0x1dc395, 0x9d8b91,
-    0x0fc2b5, 0xdec213, 0x9ee611, 0x0caf35, 0x95af81, 0x8eae31, 0xc6ae23, 0xdc8b13,
-    0x458ba7, 0xd7c283, 0x4c8b37, 0x55c387, 0x54e707, 0xd4af03, 0x4ec237, 0x46e627};
-__constant__ unsigned long long kHNext[24] = {
-    0x3148310020ull, 0x41d6a78021ull, 0x8c107109e0ull, 0xa0c4020e6cull, 0x90e3021107ull, 0x081c56b0c5ull,
-    0x5a92541cc6ull, 0x82552398e4ull, 0x2c46b32088ull, 0x7852eb3526ull, 0xbbd413d94dull, 0x0542b621abull,
-    0x9b1ac71683ull, 0xb31adadd6aull, 0xaddce5a925ull, 0x49a44789e1ull, 0x80aef84cf6ull, 0x144159c588ull,
-    0x951e2912d3ull, 0x9cecc8c2b7ull, 0xa52b790e30ull, 0x756952c46bull, 0xb4ecd4ba54ull, 0xbbaf482552ull};
 
 // keys (optional): per force workgroup of the coming force pass, the largest
 // interaction count that its waves' first bodies had in the previous force pass
@@ -1228,16 +1149,17 @@ void launch_tree_init(hipStream_t s, int nsys, const int* slots, SysState* st) {
     NB_LAUNCH(k_tree_init, (nsys + 255) / 256, 256, s, nsys, slots, st);
 }
 void launch_build(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
-                  const float4* node4, int* child, int* cparent, int* cpend, int stride, int exclude) {
+                  const float4* node4, int* child, int* cparent, int* cpend, float4* cdata, int stride,
+                  int exclude) {
     if (build_block(stride) == kBuildBlockSparse)
         NB_LAUNCH(k_build<kBuildBlockSparse>, nblk, kBuildBlockSparse, s, sd, st, work, node4, child, cparent, cpend,
-                  stride, exclude);
+                  cdata, stride, exclude);
     else if (build_block(stride) == kBuildBlockStrided)
         NB_LAUNCH(k_build<kBuildBlockStrided>, nblk, kBuildBlockStrided, s, sd, st, work, node4, child, cparent, cpend,
-                  stride, exclude);
+                  cdata, stride, exclude);
     else
-        NB_LAUNCH(k_build<kBuildBlock>, nblk, kBuildBlock, s, sd, st, work, node4, child, cparent, cpend, stride,
-                  exclude);
+        NB_LAUNCH(k_build<kBuildBlock>, nblk, kBuildBlock, s, sd, st, work, node4, child, cparent, cpend, cdata,
+                  stride, exclude);
 }
 void launch_summarize(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
                       float4* node4, const int* child, const int* cparent, int* cpend, int* ccount, int* slotoff,
