@@ -93,7 +93,11 @@ def parse_args():
     ap.add_argument("--no-shares", action="store_true", help="bh1m: skip the per-GPU target-share timing")
     ap.add_argument("--no-companion", action="store_true", help="bh1m: skip the timed run under the other rule set")
     ap.add_argument("--requests", type=int, default=2000, help="serving: number of requests")
-    ap.add_argument("--rate", type=float, default=6.0, help="serving: Poisson arrival rate, requests/s")
+    ap.add_argument("--rate", type=float, default=0.0,
+                    help="serving: Poisson arrival rate, requests/s (0: --load of the measured capacity)")
+    ap.add_argument("--load", type=float, default=0.8, help="serving: offered fraction of the measured capacity")
+    ap.add_argument("--capacity-requests", type=int, default=300,
+                    help="serving: requests of the closed-loop saturation run")
     ap.add_argument("--budget-gib", type=float, default=96.0, help="serving: HBM budget of the server arena")
     ap.add_argument("--max-systems", type=int, default=512, help="serving: system slots")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -666,18 +670,119 @@ def bench_precision(args):
     print(json.dumps(line), flush=True)
 
 
-def bench_serving(args):
-    """Continuous-batching server under open-loop Poisson arrivals.  Each request is a new
-    Plummer system (M = 1, a = 1, seed 3-derived), N log-uniform in [4096, 2,097,152], steps
-    uniform in [10, 200], theta from {0.3, 0.5, 0.7}, eps = 0.01, dt = 1e-3, Euler; brute force
-    below the autotuned crossover N, Barnes-Hut (STANDARD) from it on.  The server admits by
-    HBM budget and steps every resident system once per tick.  Latency runs from the scheduled
-    arrival time to the tick that retires the request; throughput is bodies x steps over the
-    span from the first arrival to the last completion."""
-    import queue
-    import threading
+def serving_run(sv, rq, idx, arrivals, log_every=10.0):
+    """Drive the server over the requests `idx` of the stream `rq`: each is submitted at its
+    arrival time (arrivals None: a backlog, every request submitted at time 0 with its initial
+    conditions generated before the clock starts), the server ticks while it has work.  Open-loop
+    initial conditions are generated by the client in a thread pool, ahead of use.  Returns
+    per-request submit and finish times, per-tick step times and the span."""
+    from concurrent.futures import ThreadPoolExecutor
 
     from nbodysimulation_amd import _lib as L
+    from nbodysimulation_amd import workloads as W
+
+    R = len(idx)
+    pool = ThreadPoolExecutor(max_workers=8)  # the generator releases the GIL
+    futs = {}
+
+    def next_ic(k):  # the k-th request's state, generated up to 32 requests ahead
+        for j in range(k, min(k + 32, R)):
+            if j not in futs:
+                futs[j] = pool.submit(W.serving_ic, int(idx[j]), int(rq.sizes[idx[j]]))
+        return futs.pop(k).result()
+
+    pre = [next_ic(k) for k in range(R)] if arrivals is None else None
+    ids = {}
+    submit = np.zeros(R)
+    finish = np.zeros(R)
+    tick_s = []
+    failed = nxt = done = 0
+    t0 = time.perf_counter()  # after the backlog's initial conditions exist
+    last_log = t0
+    while done < R:
+        now = time.perf_counter() - t0
+        while nxt < R and (arrivals is None or arrivals[nxt] <= now):
+            p, v, m = next_ic(nxt) if pre is None else pre[nxt]
+            if pre is not None:
+                pre[nxt] = None
+            i = idx[nxt]
+            ids[sv.submit(rq.specs[i], int(rq.steps[i]), p, v, m)] = nxt
+            submit[nxt] = time.perf_counter() - t0 if arrivals is None else arrivals[nxt]
+            nxt += 1
+            now = time.perf_counter() - t0
+        st = sv.stats()
+        if st["active"] == 0 and st["queued"] == 0 and st["finished"] == 0:
+            if nxt < R and arrivals is not None:
+                time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - t0)))
+            continue
+        ticks_before = st["ticks"]
+        tk = time.perf_counter()
+        sv.tick()
+        dt_tick = time.perf_counter() - tk
+        if sv.stats()["ticks"] > ticks_before:  # a step ran in this tick
+            tick_s.append(dt_tick)
+        while (r := sv.poll()) is not None:
+            k = ids.pop(r[0])
+            finish[k] = time.perf_counter() - t0
+            failed += r[1] != L.NB_OK
+            sv.fetch(r[0])
+            done += 1
+        if time.perf_counter() - last_log > log_every:
+            last_log = time.perf_counter()
+            st = sv.stats()
+            log(f"bench: t={last_log - t0:7.1f}s submitted {nxt} done {done} active {st['active']} "
+                f"queued {st['queued']} ticks {st['ticks']}")
+    pool.shutdown()
+    return submit, finish, np.array(tick_s), float(finish.max()), failed
+
+
+def serving_force_error(rq, cross, per_theta=2, targets=256, seed=11):
+    """FP64 force error of served requests: one force pass on the initial state of `per_theta`
+    requests per theta (each request's arithmetic is the same alone and served), against an FP64
+    direct sum on `targets` sampled bodies of each."""
+    from nbodysimulation_amd import _lib as L
+    from nbodysimulation_amd import workloads as W
+    from nbodysimulation_amd.engine import NBodySim, make_config
+    from oracle import oracle as orc
+
+    r = np.random.default_rng(seed)
+    out = {}
+    for th in W.SERVING_THETAS:
+        cand = np.flatnonzero(np.isclose(rq.thetas, th))
+        rel = []
+        for i in r.choice(cand, per_theta, replace=False):
+            sp = rq.specs[i]
+            n = sp.num_bodies
+            p, v, m = W.serving_ic(int(i), n)
+            with NBodySim(make_config(n, sp.algorithm, rules=sp.rules, law=L.LAW_PLUMMER, theta=sp.theta,
+                                      eps=sp.epsilon, G=sp.gravitational_constant, cell_capacity=2 * n + 1024)) as s:
+                s.upload(p, v, m)
+                if sp.algorithm == L.BARNES_HUT:
+                    s.force_pipeline()
+                else:
+                    s.brute_force()
+                a = s.download()[2]
+            tg = np.sort(r.choice(n, min(targets, n), replace=False)).astype(np.int32)
+            a64 = orc.direct_f64(p, m, orc.LAW_PLUMMER, sp.epsilon, sp.gravitational_constant, targets=tg)
+            rel.append(np.linalg.norm(a[tg] - a64, axis=1) / np.linalg.norm(a64, axis=1))
+        x = np.concatenate(rel)
+        out[f"theta={th}"] = {"median": float(np.median(x)), "p99": float(np.quantile(x, 0.99)),
+                              "requests": per_theta, "targets_per_request": targets}
+    out["against"] = "FP64 direct sum"
+    return out
+
+
+def bench_serving(args):
+    """Continuous-batching server on the request stream: each request a new Plummer system
+    (M = 1, a = 1, seed 3-derived), N log-uniform in [4096, 2,097,152], steps uniform in
+    [10, 200], theta from {0.3, 0.5, 0.7}, eps = 0.01, dt = 1e-3, Euler; brute force below the
+    autotuned crossover N, Barnes-Hut (STANDARD) from it on.  Two runs on one server:
+    (1) saturation: the first --capacity-requests requests as a backlog (all submitted at
+    once), whose bodies x steps per second is the server's capacity, `value`; (2) the open-loop
+    Poisson stream of --requests requests at --rate requests/s (default: the rate that offers
+    --load of the measured capacity), with latency from the scheduled arrival to the tick that
+    retires the request.  Both report per-tick step time percentiles (host wall time of the
+    ticks that ran a step, admission uploads and retirement downloads included)."""
     from nbodysimulation_amd import autotune, device_info
     from nbodysimulation_amd import workloads as W
     from nbodysimulation_amd.engine import NBodyServer
@@ -687,80 +792,60 @@ def bench_serving(args):
         raise SystemExit("no HIP device")
     tune = autotune.load_or_tune(0.5, 0.01, log=log)
     cross = int(tune["crossover_n"])
-    R, rate = args.requests, args.rate
-    rq = W.serving_requests(R, rate, cross)
-    sizes, steps, arrivals, specs = rq.sizes, rq.steps, rq.arrivals, rq.specs
-    work = float((sizes * steps).sum())
-    log(f"bench: serving {R} requests at {rate} req/s, {work:.3e} body-steps, budget {args.budget_gib} GiB, "
-        f"{args.max_systems} slots, crossover N={cross}")
+    R = args.requests
+    rq0 = W.serving_requests(R, 1.0, cross)  # arrival times are rescaled below
+    mean_work = float((rq0.sizes * rq0.steps).mean())
+    budget = int(args.budget_gib * (1 << 30))
 
-    # the client's initial conditions, generated ahead of arrival in a producer thread
-    ready = queue.Queue(maxsize=48)
+    def pct(x):
+        return {"p50": float(np.quantile(x, 0.5)), "p99": float(np.quantile(x, 0.99)), "max": float(x.max()),
+                "mean": float(x.mean()), "count": int(x.size)} if len(x) else None
 
-    def produce():
-        for i in range(R):
-            ready.put(W.serving_ic(i, int(sizes[i])))
-
-    threading.Thread(target=produce, daemon=True).start()
-
-    sv = NBodyServer(int(args.budget_gib * (1 << 30)), args.max_systems)
-    ids = {}
-    finish = np.zeros(R)
-    submit_lag = np.zeros(R)
-    failed = 0
-    nxt = done = 0
-    busy = 0.0
-    t0 = time.perf_counter()
-    last_log = t0
-    while done < R:
-        now = time.perf_counter() - t0
-        while nxt < R and arrivals[nxt] <= now:
-            p, v, m = ready.get()
-            ids[sv.submit(specs[nxt], int(steps[nxt]), p, v, m)] = nxt
-            submit_lag[nxt] = time.perf_counter() - t0 - arrivals[nxt]
-            nxt += 1
-            now = time.perf_counter() - t0
-        st = sv.stats()
-        if st["active"] == 0 and st["queued"] == 0 and st["finished"] == 0:
-            if nxt < R:
-                time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - t0)))
-            continue
-        tk = time.perf_counter()
-        sv.tick()
-        busy += time.perf_counter() - tk
-        while (r := sv.poll()) is not None:
-            i = ids.pop(r[0])
-            finish[i] = time.perf_counter() - t0
-            failed += r[1] != L.NB_OK
-            sv.fetch(r[0])
-            done += 1
-        if time.perf_counter() - last_log > 10:
-            last_log = time.perf_counter()
-            st = sv.stats()
-            log(f"bench: t={last_log - t0:7.1f}s submitted {nxt} done {done} active {st['active']} "
-                f"queued {st['queued']} ticks {st['ticks']}")
-    stats = sv.stats()
-    sv.close()
-    lat = finish - arrivals
-    span = float(finish.max())
+    with NBodyServer(budget, args.max_systems) as sv:
+        # (1) saturation throughput, closed loop
+        K = min(args.capacity_requests, R)
+        log(f"bench: serving capacity run, a backlog of {K} requests, budget {args.budget_gib} GiB, "
+            f"{args.max_systems} slots, crossover N={cross}")
+        s0 = sv.stats()
+        _, _, ticks_c, span_c, failed_c = serving_run(sv, rq0, list(range(K)), None)
+        s1 = sv.stats()
+        capacity = (s1["body_steps"] - s0["body_steps"]) / span_c
+        log(f"bench: capacity {capacity:.3e} bodies*steps/s over {span_c:.1f}s, tick p50 "
+            f"{1e3 * np.quantile(ticks_c, 0.5):.1f} ms p99 {1e3 * np.quantile(ticks_c, 0.99):.1f} ms")
+        # (2) open-loop Poisson arrivals at a stated fraction of the capacity
+        rate = args.rate if args.rate else args.load * capacity / mean_work
+        rq = W.serving_requests(R, rate, cross)
+        log(f"bench: serving {R} requests open-loop at {rate:.2f} req/s "
+            f"(offered {rate * mean_work:.3e} bodies*steps/s = {rate * mean_work / capacity:.2f} of capacity)")
+        s2 = sv.stats()
+        sub, fin, ticks_o, span_o, failed_o = serving_run(sv, rq, list(range(R)), rq.arrivals)
+        s3 = sv.stats()
+        arena = s3["arena_bytes"]
+    lat = fin - rq.arrivals
+    served = (s3["body_steps"] - s2["body_steps"]) / span_o
+    err = serving_force_error(rq, cross)
     line = {
-        "metric": "serving bodies*steps/s with p50/p99 request latency (continuous batching)",
-        "value": stats["body_steps"] / span, "unit": "bodies*steps/s", "n_gpus": 1, "steps": int(steps.sum()),
-        "warmup": 0, "ms_per_step": 1e3 * span / max(stats["ticks"], 1), "higher_is_better": True,
+        "metric": "serving capacity, bodies*steps/s (continuous batching), with p50/p99 request and step latency",
+        "value": capacity, "unit": "bodies*steps/s", "n_gpus": 1, "steps": int(rq.steps.sum()),
+        "warmup": 0, "ms_per_step": 1e3 * float(np.quantile(ticks_o, 0.5)), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic: seeded Plummer spheres generated by the client while the server runs",
-        "config": {"workload": f"{R} requests, Poisson {rate} req/s, N log-uniform [4096, 2097152], steps U[10,200], "
-                               f"theta in {{0.3,0.5,0.7}}, seed 3", "hbm_budget_bytes": int(args.budget_gib * (1 << 30)),
-                   "arena_bytes": stats["arena_bytes"], "max_systems": args.max_systems, "crossover_n": cross,
-                   "device": info},
-        "latency_s": {"p50": float(np.quantile(lat, 0.5)), "p99": float(np.quantile(lat, 0.99)),
-                      "max": float(lat.max()), "mean": float(lat.mean())},
-        "span_s": span, "ticks": stats["ticks"], "tick_busy_frac": busy / span, "failed": int(failed),
-        "client_submit_lag_s": {"p50": float(np.quantile(submit_lag, 0.5)), "max": float(submit_lag.max())},
-        "offered_body_steps_per_s": work / float(arrivals[-1]) if R > 1 else None,
+        "config": {"workload": f"{R} requests, Poisson arrivals, N log-uniform [4096, 2097152], steps U[10,200], "
+                               f"theta in {{0.3,0.5,0.7}}, seed 3", "hbm_budget_bytes": budget, "arena_bytes": arena,
+                   "max_systems": args.max_systems, "crossover_n": cross, "device": info},
+        "capacity": {"body_steps_per_s": capacity, "requests": K, "span_s": span_c, "failed": int(failed_c),
+                     "tick_step_s": pct(ticks_c),
+                     "how": "backlog: the first requests of the stream all submitted at time 0 (initial conditions "
+                            "generated before the clock); bodies x steps over the time to finish them all"},
+        "open_loop": {"rate_req_per_s": rate, "offered_body_steps_per_s": rate * mean_work,
+                      "offered_fraction_of_capacity": rate * mean_work / capacity,
+                      "served_body_steps_per_s": served, "span_s": span_o, "failed": int(failed_o),
+                      "latency_s": pct(lat), "tick_step_s": pct(ticks_o)},
+        "force_error": err,
     }
-    log(f"bench: serving {line['value']:.3e} bodies*steps/s, latency p50 {line['latency_s']['p50']:.2f}s "
-        f"p99 {line['latency_s']['p99']:.2f}s, span {span:.1f}s, ticks {stats['ticks']}")
+    log(f"bench: open loop served {served:.3e} bodies*steps/s, latency p50 {line['open_loop']['latency_s']['p50']:.2f}s "
+        f"p99 {line['open_loop']['latency_s']['p99']:.2f}s; tick p50 {1e3 * np.quantile(ticks_o, 0.5):.1f} ms "
+        f"p99 {1e3 * np.quantile(ticks_o, 0.99):.1f} ms")
     print(json.dumps(line), flush=True)
 
 

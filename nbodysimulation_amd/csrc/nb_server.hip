@@ -1,15 +1,21 @@
 // nb_server.hip -- continuous-batching N-body server over one HBM arena.
 //
-// Requests (a system plus a number of steps) are queued on submit.  Every tick
-// admits queued requests in arrival order while system slots and arena ranges
-// (nodes, bodies, cells: the HBM budget) are free, runs one batched step of
-// every resident system, and retires the systems that have finished their
-// steps (or failed), staging their final state on the host and returning their
-// ranges.  Systems therefore join while others are mid-run and share every
-// launch; each system's arithmetic is independent of its neighbours, so its
-// result is bit-identical to running it alone.
+// Requests (a system plus a number of steps) are queued on submit.  A tick is a
+// pipeline: (1) the requests staged by the previous tick join the resident
+// systems; (2) one batched step of every resident system is launched on the
+// compute stream; (3) while it runs, queued requests are admitted in arrival
+// order while system slots and arena ranges (nodes, bodies, cells: the HBM
+// budget) are free, packed into a library-owned pinned staging buffer and
+// uploaded on a separate copy stream into ranges the running step does not
+// touch (they join at the next tick); (4) the tick waits for the step and
+// retires the systems that have finished their steps (or failed), downloading
+// their final state through a pinned buffer and returning their ranges.
+// Systems therefore join while others are mid-run and share every launch; each
+// system's arithmetic is independent of its neighbours, so its result is
+// bit-identical to running it alone.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <deque>
@@ -84,6 +90,16 @@ struct nb_server {
     std::vector<int> free_slots;
     std::deque<std::unique_ptr<Request>> queue;
     std::vector<std::unique_ptr<Request>> active;
+    std::vector<std::unique_ptr<Request>> staged;  // uploaded during the last tick; join at the next
+    // admission uploads: a copy stream and two pinned staging halves used alternately
+    // (one is packed while the other's copy may still run), each with its copy's event
+    hipStream_t up = nullptr;
+    float4* stage[2] = {};
+    size_t stage_cap = 0;  // float4 per half
+    hipEvent_t stage_ev[2] = {};
+    int stage_half = 0;
+    float4* dl = nullptr;  // pinned buffer of retirement downloads
+    size_t dl_cap = 0;     // float4
     std::deque<int64_t> finished_order;
     std::map<int64_t, std::unique_ptr<Request>> finished;
     int64_t next_id = 1;
@@ -141,44 +157,91 @@ static void unplace(nb_server* sv, Request& r) {
     sv->free_slots.push_back(r.slot.slot);
 }
 
-static nb_status upload_request(nb_server* sv, Request& r) {
+constexpr size_t kStateF4 = sizeof(nb::SysState) / sizeof(float4);
+static_assert(sizeof(nb::SysState) % sizeof(float4) == 0, "SysState fills whole float4s");
+
+// Pinned host memory of at least `want` float4 in *buf (grown, never shrunk).
+static nb_status ensure_pinned(float4** buf, size_t* cap, size_t want) {
+    if (*cap >= want) return NB_OK;
+    if (*buf) (void)hipHostFree(*buf);
+    *buf = nullptr;
+    *cap = 0;
+    const size_t n = std::max(want, (size_t)1 << 16);
+    NB_HIPS(hipHostMalloc(reinterpret_cast<void**>(buf), sizeof(float4) * n, hipHostMallocDefault));
+    *cap = n;
+    return NB_OK;
+}
+
+// Queue the upload of a placed request on the copy stream: a fresh slot state, its
+// cell slots cleared, positions+masses and velocities, zero accelerations.  The
+// source is one half of the pinned staging buffer; the half's previous copy is
+// waited for first.  Nothing the running step reads is written.
+static nb_status stage_request(nb_server* sv, Request& r) {
     nb::Engine& g = sv->eng;
-    nb_status st = g.reset_slot(r.slot);
-    if (st) return st;
     const int n = r.spec.n;
-    if (n == 0) return NB_OK;
-    std::vector<float4> h4(n), hv(n);
+    const size_t want = 2 * (size_t)n + kStateF4;
+    if (want > sv->stage_cap) {  // grow both halves: their copies must be done first
+        NB_HIPS(hipStreamSynchronize(sv->up));
+        for (int h = 0; h < 2; ++h) {
+            size_t cap = sv->stage_cap;
+            nb_status st = ensure_pinned(&sv->stage[h], &cap, want);
+            if (st) return st;
+            if (h == 1) sv->stage_cap = cap;
+        }
+    }
+    const int half = sv->stage_half;
+    sv->stage_half ^= 1;
+    NB_HIPS(hipEventSynchronize(sv->stage_ev[half]));
+    float4* h4 = sv->stage[half];
+    float4* hv = h4 + n;
+    nb::SysState* z = reinterpret_cast<nb::SysState*>(hv + n);
+    std::memset(z, 0, sizeof(*z));
+    z->ncells = 1;
+    z->max_depth = 1;
     for (int i = 0; i < n; ++i) {
         h4[i] = make_float4(r.pos[3 * i], r.pos[3 * i + 1], r.pos[3 * i + 2], r.mass[i]);
         hv[i] = make_float4(r.vel[3 * i], r.vel[3 * i + 1], r.vel[3 * i + 2], 0.0f);
     }
-    NB_HIPS(hipMemcpyAsync(g.node4 + r.slot.node_off, h4.data(), sizeof(float4) * n, hipMemcpyHostToDevice, g.stream));
-    NB_HIPS(hipMemcpyAsync(g.vel4 + r.slot.body_off, hv.data(), sizeof(float4) * n, hipMemcpyHostToDevice, g.stream));
-    NB_HIPS(hipMemsetAsync(g.acc4 + r.slot.body_off, 0, sizeof(float4) * n, g.stream));
-    NB_HIPS(hipStreamSynchronize(g.stream));
+    const nb::Slot& sl = r.slot;
+    if (r.spec.algo == NB_BARNES_HUT && r.spec.cap > 0) {
+        NB_HIPS(hipMemsetAsync(g.child + 8 * sl.cell_off, 0xff, sizeof(int) * 8 * (size_t)r.spec.cap, sv->up));
+        NB_HIPS(hipMemsetAsync(g.cpend + sl.cell_off, 0, sizeof(int) * (size_t)r.spec.cap, sv->up));
+    }
+    NB_HIPS(hipMemcpyAsync(g.d_state + sl.slot, z, sizeof(*z), hipMemcpyHostToDevice, sv->up));
+    if (n > 0) {
+        NB_HIPS(hipMemcpyAsync(g.node4 + sl.node_off, h4, sizeof(float4) * n, hipMemcpyHostToDevice, sv->up));
+        NB_HIPS(hipMemcpyAsync(g.vel4 + sl.body_off, hv, sizeof(float4) * n, hipMemcpyHostToDevice, sv->up));
+        NB_HIPS(hipMemsetAsync(g.acc4 + sl.body_off, 0, sizeof(float4) * n, sv->up));
+    }
+    NB_HIPS(hipEventRecord(sv->stage_ev[half], sv->up));
     return NB_OK;
 }
 
+// The final state of a retired request, through the pinned download buffer.
 static nb_status download_request(nb_server* sv, Request& r) {
     nb::Engine& g = sv->eng;
     const int n = r.spec.n;
     r.acc.assign(3 * (size_t)n, 0.0f);
     if (n == 0) return NB_OK;
-    std::vector<float4> h(n);
-    auto get = [&](const float4* src, std::vector<float>& dst) -> nb_status {
-        NB_HIPS(hipMemcpyAsync(h.data(), src, sizeof(float4) * n, hipMemcpyDeviceToHost, g.stream));
-        NB_HIPS(hipStreamSynchronize(g.stream));
+    nb_status st = ensure_pinned(&sv->dl, &sv->dl_cap, 3 * (size_t)n);
+    if (st) return st;
+    float4* h = sv->dl;
+    NB_HIPS(hipMemcpyAsync(h, g.node4 + r.slot.node_off, sizeof(float4) * n, hipMemcpyDeviceToHost, g.stream));
+    NB_HIPS(hipMemcpyAsync(h + n, g.vel4 + r.slot.body_off, sizeof(float4) * n, hipMemcpyDeviceToHost, g.stream));
+    NB_HIPS(hipMemcpyAsync(h + 2 * (size_t)n, g.acc4 + r.slot.body_off, sizeof(float4) * n, hipMemcpyDeviceToHost,
+                           g.stream));
+    NB_HIPS(hipStreamSynchronize(g.stream));
+    std::vector<float>* dst[3] = {&r.pos, &r.vel, &r.acc};
+    for (int a = 0; a < 3; ++a) {
+        const float4* src = h + (size_t)a * n;
+        float* o = dst[a]->data();
         for (int i = 0; i < n; ++i) {
-            dst[3 * i] = h[i].x;
-            dst[3 * i + 1] = h[i].y;
-            dst[3 * i + 2] = h[i].z;
+            o[3 * i] = src[i].x;
+            o[3 * i + 1] = src[i].y;
+            o[3 * i + 2] = src[i].z;
         }
-        return NB_OK;
-    };
-    nb_status st = get(g.node4 + r.slot.node_off, r.pos);
-    if (!st) st = get(g.vel4 + r.slot.body_off, r.vel);
-    if (!st) st = get(g.acc4 + r.slot.body_off, r.acc);
-    return st;
+    }
+    return NB_OK;
 }
 
 static void enqueue_force(nb::Engine& g) {
@@ -217,8 +280,17 @@ nb_status nb_server_create(int64_t hbm_budget_bytes, int32_t max_systems, int32_
     sv->law = law;
     sv->accum = accumulation;
     nb_status st = sv->eng.reserve(c, law, accumulation == NB_ACCUM_KAHAN, stream);
+    if (!st && hipStreamCreateWithFlags(&sv->up, hipStreamNonBlocking) != hipSuccess) {
+        sv->up = nullptr;
+        st = sfail(NB_E_HIP, "hipStreamCreateWithFlags(copy stream) failed");
+    }
+    for (int h = 0; h < 2 && !st; ++h)
+        if (hipEventCreateWithFlags(&sv->stage_ev[h], hipEventDisableTiming) != hipSuccess) {
+            sv->stage_ev[h] = nullptr;
+            st = sfail(NB_E_HIP, "hipEventCreateWithFlags failed");
+        }
     if (st) {
-        delete sv;
+        nb_server_destroy(sv);
         return st;
     }
     sv->nodes.init(c.nodes);
@@ -232,6 +304,13 @@ nb_status nb_server_create(int64_t hbm_budget_bytes, int32_t max_systems, int32_
 nb_status nb_server_destroy(nb_server* sv) {
     if (!sv) return NB_OK;
     if (sv->eng.stream) (void)hipStreamSynchronize(sv->eng.stream);
+    if (sv->up) (void)hipStreamSynchronize(sv->up);
+    for (int h = 0; h < 2; ++h) {
+        if (sv->stage_ev[h]) (void)hipEventDestroy(sv->stage_ev[h]);
+        if (sv->stage[h]) (void)hipHostFree(sv->stage[h]);
+    }
+    if (sv->dl) (void)hipHostFree(sv->dl);
+    if (sv->up) (void)hipStreamDestroy(sv->up);
     delete sv;
     return NB_OK;
 }
@@ -269,27 +348,16 @@ nb_status nb_server_tick(nb_server* sv, int32_t* completed) {
     if (completed) *completed = 0;
     int done = 0;
     nb::Engine& g = sv->eng;
+    // (1) the requests staged by the last tick join (their uploads were queued on the
+    // copy stream while that tick's step ran)
     std::vector<nb::Slot> fresh;
-    // admit in arrival order while a slot and arena room are free
-    while (!sv->queue.empty()) {
-        Request& r = *sv->queue.front();
-        if (r.steps_left == 0) {  // nothing to run: retired on arrival with its input state
-            r.acc.assign(3 * (size_t)r.spec.n, 0.0f);
-            sv->finished_order.push_back(r.id);
-            sv->finished[r.id] = std::move(sv->queue.front());
-            sv->queue.pop_front();
-            ++done;
-            continue;
+    if (!sv->staged.empty()) {
+        NB_HIPS(hipStreamSynchronize(sv->up));
+        for (auto& r : sv->staged) {
+            fresh.push_back(r->slot);
+            sv->active.push_back(std::move(r));
         }
-        if (!try_place(sv, r)) break;
-        nb_status st = upload_request(sv, r);
-        if (st) {  // the request stays queued; its ranges go back to the pools
-            unplace(sv, r);
-            return st;
-        }
-        fresh.push_back(r.slot);
-        sv->active.push_back(std::move(sv->queue.front()));
-        sv->queue.pop_front();
+        sv->staged.clear();
         sv->dirty = true;
     }
     if (sv->method == NB_LEAPFROG && !fresh.empty()) {
@@ -306,13 +374,40 @@ nb_status nb_server_tick(nb_server* sv, int32_t* completed) {
         if (st) return st;
         sv->dirty = false;
     }
-    if (!sv->active.empty()) {
+    // (2) one step of every resident system, asynchronously
+    const bool stepping = !sv->active.empty();
+    if (stepping) {
         const bool lf = sv->method == NB_LEAPFROG;
         if (lf) g.update(2);
         enqueue_force(g);
         g.update(lf ? 1 : 0);
         if (g.any_collision) g.collide();  // the requests that ask for it, after the integrator
         NB_HIPS(hipGetLastError());
+    }
+    // (3) admit in arrival order while a slot and arena room are free; the uploads run
+    // on the copy stream beside the step
+    while (!sv->queue.empty()) {
+        Request& r = *sv->queue.front();
+        if (r.steps_left == 0) {  // nothing to run: retired on arrival with its input state
+            r.acc.assign(3 * (size_t)r.spec.n, 0.0f);
+            sv->finished_order.push_back(r.id);
+            sv->finished[r.id] = std::move(sv->queue.front());
+            sv->queue.pop_front();
+            ++done;
+            continue;
+        }
+        if (!try_place(sv, r)) break;
+        nb_status st = stage_request(sv, r);
+        if (st) {  // the request stays queued; its ranges go back to the pools
+            unplace(sv, r);
+            if (stepping) (void)hipStreamSynchronize(g.stream);
+            return st;
+        }
+        sv->staged.push_back(std::move(sv->queue.front()));
+        sv->queue.pop_front();
+    }
+    // (4) wait for the step, then retire
+    if (stepping) {
         std::vector<nb::SysState> s;
         nb_status st = g.download_state(s);
         if (st) return st;
@@ -372,7 +467,7 @@ nb_status nb_server_fetch(nb_server* sv, int64_t id, float* pos, float* vel, flo
 
 nb_status nb_server_stats_get(nb_server* sv, nb_server_stats* out) {
     if (!sv || !out) return sfail(NB_E_INVALID_ARG, "bad arguments");
-    out->active = (int32_t)sv->active.size();
+    out->active = (int32_t)(sv->active.size() + sv->staged.size());  // staged requests join at the next tick
     out->queued = (int32_t)sv->queue.size();
     out->finished = (int32_t)sv->finished.size();
     out->reserved = 0;
