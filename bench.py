@@ -100,8 +100,9 @@ def parse_args():
                     help="serving: requests of the closed-loop saturation run")
     ap.add_argument("--budget-gib", type=float, default=96.0, help="serving: HBM budget of the server arena")
     ap.add_argument("--max-systems", type=int, default=512, help="serving: system slots")
-    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
-                    help="first run a plain device copy of this many MiB (scale for HBM byte counters)")
+    ap.add_argument("--calibrate-copy-mib", type=lambda t: [int(x) for x in t.split(",") if x], default=[],
+                    help="first run plain device copies of these many MiB, comma-separated (scale for HBM "
+                         "byte counters)")
     return ap.parse_args()
 
 
@@ -117,8 +118,8 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (each box has one MI355X); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    if args.calibrate_copy_mib:
-        calibrate_copy(args.calibrate_copy_mib)
+    for mib in args.calibrate_copy_mib:
+        calibrate_copy(mib)
     {"bh1m": bench_bh, "ensemble": bench_ensemble, "precision": bench_precision, "serving": bench_serving,
      "bf4096": bench_bf4096}[args.workload](args)
 
@@ -205,7 +206,7 @@ def reference_vote_error(pos, mass, theta, eps, G, engine_acc, groups=8, seed=99
     return out
 
 
-def cpu_step(pos, vel, mass, rules, theta, eps, G, dt, cap, sample1):
+def cpu_step(pos, vel, mass, rules, theta, eps, G, dt, cap, sample1, one_thread=True):
     """The oracle's Barnes-Hut step timed on this host: bounds, the sequential tree build with
     summarize, sort and force order (serial code), the per-body traversal of every body (OpenMP
     over all threads) and the Euler update, each part measured whole.  Also the per-body
@@ -227,6 +228,8 @@ def cpu_step(pos, vel, mass, rules, theta, eps, G, dt, cap, sample1):
     parts = {"bounds": t[1] - t[0], "build_summarize_sort_serial": t[2] - t[1], "traversal": t[3] - t[2],
              "update": t[4] - t[3]}
     full = t[4] - t[0]
+    if not one_thread:
+        return {"full_s": full, "parts_s": parts, "threads": threads, "i_alg": float(ni.sum())}
     orc.set_threads(1)
     try:
         u = [time.perf_counter()]
@@ -311,7 +314,8 @@ def bench_bh(args):
                         for k, v in ref_vote.items() if k.startswith("engine_")))
 
     # the oracle's step on the same state: the CPU baseline, and the per-body interaction count I_alg
-    c = cpu_step(p_end, v_end, mass, rules, theta, eps, G, dt, tinfo["capacity"], min(args.cpu_sample, n))
+    c = cpu_step(p_end, v_end, mass, rules, theta, eps, G, dt, tinfo["capacity"], min(args.cpu_sample, n),
+                 one_thread=not args.no_cpu_baseline)
     i_alg = c["i_alg"]
     cpu = None
     if not args.no_cpu_baseline:
