@@ -34,7 +34,9 @@ void launch_sort(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const
                  const int* child, const int* slotoff, int* sorted, int* forder, const float4* acc4, int* keys,
                  const int* sorder);
 void launch_force_order(hipStream_t s, const BlkWork* in, int len, int* keys, BlkWork* out);
-void launch_force_bh(hipStream_t s, int nblk, bool kahan, const SysDesc* sd, SysState* st,
+// pair: the instance for systems under REFERENCE rules (walks two cells per scalar-load
+// round trip); otherwise the one for STANDARD rules.  Each skips the other's systems.
+void launch_force_bh(hipStream_t s, int nblk, bool kahan, bool pair, const SysDesc* sd, SysState* st,
                      const BlkWork* work, const float4* node4, const int* child, const float4* cdata,
                      const int* order, float4* acc4, int* sorder);
 // mode >= 0: also integrate (k_update's mode 0 or 1); mode 0 writes the moved
@@ -111,6 +113,7 @@ struct Engine {
     std::vector<SysDesc> hdesc;  // indexed by slot
     std::vector<Slot> active;
     bool any_bh = false, any_bf = false;
+    bool any_bh_ref = false, any_bh_std = false;  // Barnes-Hut systems under REFERENCE / STANDARD rules
     int bf_tb = 1;  // brute-force targets per lane of the current batch (1 or 2)
     int law = 1;
     bool kahan = false;

@@ -192,6 +192,7 @@ nb_status Engine::activate(const std::vector<Slot>& slots) {
     std::vector<char> seen(capc.systems, 0);
     long long bf_waves2 = 0;  // brute-force waves of the batch at two targets per lane
     any_bh = any_bf = any_collision = false;
+    any_bh_ref = any_bh_std = false;
     for (const Slot& sl : slots) {
         if (sl.slot < 0 || sl.slot >= capc.systems || seen[sl.slot]) {
             set_error("slot index out of range or used twice");
@@ -232,6 +233,7 @@ nb_status Engine::activate(const std::vector<Slot>& slots) {
         ids.push_back(s);
         if (bh && d.n > 0) {
             any_bh = true;
+            (d.rules == 0 ? any_bh_ref : any_bh_std) = true;
             d.bbox_first = (int)t[T_BOUNDS].size();
             add_ranges(t[T_BOUNDS], s, d.n, kBoundsPerBlock);
             d.bbox_blocks = (int)t[T_BOUNDS].size() - d.bbox_first;
@@ -354,8 +356,10 @@ void Engine::sort() {
 void Engine::force() {
     TraceRange tr("nb:force");
     if (lpt) launch_force_order(stream, tab[T_GROUPS], tab_len[T_GROUPS], gkey, tab[T_FORDER]);
-    launch_force_bh(stream, tab_len[T_GROUPS], kahan, d_desc, d_state, tab[lpt ? T_FORDER : T_GROUPS], node4, child,
-                    cdata, forder, acc4, sorder);
+    for (int pair = 0; pair < 2; ++pair)
+        if (pair ? any_bh_ref : any_bh_std)
+            launch_force_bh(stream, tab_len[T_GROUPS], kahan, pair != 0, d_desc, d_state,
+                            tab[lpt ? T_FORDER : T_GROUPS], node4, child, cdata, forder, acc4, sorder);
 }
 void Engine::force_range(int sys, int lo, int hi) {
     TraceRange tr("nb:force_range");
@@ -364,7 +368,8 @@ void Engine::force_range(int sys, int lo, int hi) {
     const int stop = (int)std::min<long long>(hdesc[sys].n, ((long long)hi + kForceVote - 1) / kForceVote * kForceVote);
     const int nblk = (stop - start + per - 1) / per;
     launch_range_table(stream, tab[T_RANGE], nblk, sys, start, stop, lo, hi, per);
-    launch_force_bh(stream, nblk, kahan, d_desc, d_state, tab[T_RANGE], node4, child, cdata, forder, acc4, nullptr);
+    launch_force_bh(stream, nblk, kahan, hdesc[sys].rules == 0, d_desc, d_state, tab[T_RANGE], node4, child, cdata,
+                    forder, acc4, nullptr);
 }
 void Engine::brute_range(int sys, int lo, int hi) {
     TraceRange tr("nb:brute_force_range");

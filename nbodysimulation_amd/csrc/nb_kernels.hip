@@ -592,21 +592,41 @@ __global__ __launch_bounds__(256) void k_sort(const SysDesc* __restrict__ sd, Sy
 // which the build keeps as plain scalar branches.  acc = G * sum d*m*rsqrt(d2)^3.
 
 constexpr int kStack = 8 * kDqLen;
+// Paired walk: each iteration pops the two top entries A (top) and B and fetches
+// both cells with one scalar-load round trip, then pushes B's opened children and A's on
+// top of them.  Entry levels stay non-decreasing from the bottom of the stack up (B's
+// level <= A's, and what lay below B is no deeper than B), so a level-L entry is pushed
+// only when a level-(L-1) cell is one of the two popped, i.e. when at most one entry of
+// level >= L is on the stack: at most 1 + 16 entries of any level, 17 * 65 in all.
+// The same cells are accepted as by the one-cell walk (acceptance depends only on the
+// cell and the group), in a different order.
+constexpr int kStackPair = 17 * kDqLen;
+// Which systems walk in pairs is a property of the system alone (its rule set), so that
+// a system's results do not depend on the batch it is in: REFERENCE-rule passes are bound
+// by the scalar-load latency of their longest walks and walk in pairs (1M: 2.69 -> 2.33 ms);
+// STANDARD passes are bound by VALU issue and keep the one-cell walk, in a kernel instance
+// of their own (compiled into one kernel with the paired walk, the register and LDS
+// footprint of the pairs slowed them 1.50 -> 1.76 ms).  A launch of one instance skips the
+// workgroups of the other rule set; a batch that holds both runs both instances.
+
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef float v16f __attribute__((ext_vector_type(16)));
 
 // sorder (full passes only, else NULL): receives the force order, the order in
 // which the next sort visits the bodies.
-template <bool KAHAN>
+template <bool KAHAN, bool PAIR>
 __global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __restrict__ sd, SysState* st,
                                                   const BlkWork* __restrict__ work,
                                                   const float4* __restrict__ node4, const int* __restrict__ child,
                                                   const float4* __restrict__ cdata, const int* __restrict__ sorted,
                                                   float4* __restrict__ acc4, int* __restrict__ sorder) {
-    __shared__ int stk[kForceWaves][kStack];
+    constexpr int kS = PAIR ? kStackPair : kStack;
+    __shared__ int stk[kForceWaves][kS];
     __shared__ float dqs[kDqLen];
     const BlkWork w = work[blockIdx.x];
     const SysDesc d = sd[w.sys];
     const SysState* S = st + w.sys;
-    if (S->status) return;
+    if (S->status || (d.rules == 0) != PAIR) return;  // the other instance's system
     for (int i = threadIdx.x; i < kDqLen; i += blockDim.x) dqs[i] = S->dq[i];
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -629,62 +649,88 @@ __global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __
     while (sp > 0) {
         --sp;
         const int cur = __builtin_amdgcn_readfirstlane(sk[sp]);  // (cell << 7) | level
-        const int node = cur >> 7;
-        const int lvl = min(cur & 127, kMaxLevel);
-        const float dql = dqs[lvl];
-        // The cell's 8 slots (32 bytes) and its children's (x,y,z,m) (one 128-byte
-        // block) in one batch of wave-uniform scalar loads with a single wait, so that
-        // a popped cell costs one scalar-memory round trip.  Left to itself the
-        // compiler sinks each child's load into the branch that uses it and waits up to
-        // three times per cell, which lengthens the walk of the longest waves.  The
-        // wait is inside the asm block: no load is in flight when it ends.
-        typedef int v8i __attribute__((ext_vector_type(8)));
-        typedef float v16f __attribute__((ext_vector_type(16)));
-        v8i sv;
-        v16f q0, q1;
-        asm volatile(
-            "s_load_dwordx8 %0, %3, 0x0\n\t"
-            "s_load_dwordx16 %1, %4, 0x0\n\t"
-            "s_load_dwordx16 %2, %4, 0x40\n\t"
-            "s_waitcnt lgkmcnt(0)"
-            : "=&s"(sv), "=&s"(q0), "=&s"(q1)
-            : "s"(C + 8 * node), "s"(CD + 8 * node)
-            : "memory");
-        const int vs[8] = {sv[0], sv[1], sv[2], sv[3], sv[4], sv[5], sv[6], sv[7]};
-        float4 qs[8];
-        qs[0] = make_float4(q0[0], q0[1], q0[2], q0[3]);
-        qs[1] = make_float4(q0[4], q0[5], q0[6], q0[7]);
-        qs[2] = make_float4(q0[8], q0[9], q0[10], q0[11]);
-        qs[3] = make_float4(q0[12], q0[13], q0[14], q0[15]);
-        qs[4] = make_float4(q1[0], q1[1], q1[2], q1[3]);
-        qs[5] = make_float4(q1[4], q1[5], q1[6], q1[7]);
-        qs[6] = make_float4(q1[8], q1[9], q1[10], q1[11]);
-        qs[7] = make_float4(q1[12], q1[13], q1[14], q1[15]);
+        // A popped cell's 8 slots (32 bytes) and its children's (x,y,z,m) (one 128-byte
+        // block) come in one batch of wave-uniform scalar loads with a single wait, so
+        // that a cell (or a pair of cells) costs one scalar-memory round trip.  Left to
+        // itself the compiler sinks each child's load into the branch that uses it and
+        // waits up to three times per cell, which lengthens the walk of the longest
+        // waves.  The wait is inside the asm block: no load is in flight when it ends.
+        // Cell 0 of the batch is visited first: with a pair, the second-popped cell.
+        v8i sv[2];
+        v16f q0[2], q1[2];
+        int lv[2];
+        int cells = 1;
+        if (PAIR && sp > 0) {
+            --sp;
+            const int cur2 = __builtin_amdgcn_readfirstlane(sk[sp]);
+            const int na = cur >> 7, nb2 = cur2 >> 7;
+            lv[0] = min(cur2 & 127, kMaxLevel);
+            lv[1] = min(cur & 127, kMaxLevel);
+            cells = 2;
+            asm volatile(
+                "s_load_dwordx8 %0, %6, 0x0\n\t"
+                "s_load_dwordx16 %1, %7, 0x0\n\t"
+                "s_load_dwordx16 %2, %7, 0x40\n\t"
+                "s_load_dwordx8 %3, %8, 0x0\n\t"
+                "s_load_dwordx16 %4, %9, 0x0\n\t"
+                "s_load_dwordx16 %5, %9, 0x40\n\t"
+                "s_waitcnt lgkmcnt(0)"
+                : "=&s"(sv[0]), "=&s"(q0[0]), "=&s"(q1[0]), "=&s"(sv[1]), "=&s"(q0[1]), "=&s"(q1[1])
+                : "s"(C + 8 * nb2), "s"(CD + 8 * nb2), "s"(C + 8 * na), "s"(CD + 8 * na)
+                : "memory");
+        } else {
+            const int na = cur >> 7;
+            lv[0] = min(cur & 127, kMaxLevel);
+            asm volatile(
+                "s_load_dwordx8 %0, %3, 0x0\n\t"
+                "s_load_dwordx16 %1, %4, 0x0\n\t"
+                "s_load_dwordx16 %2, %4, 0x40\n\t"
+                "s_waitcnt lgkmcnt(0)"
+                : "=&s"(sv[0]), "=&s"(q0[0]), "=&s"(q1[0])
+                : "s"(C + 8 * na), "s"(CD + 8 * na)
+                : "memory");
+        }
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int v = vs[k];
-            if (v < 0) continue;
-            const float dx = qs[k].x - p.x, dy = qs[k].y - p.y, dz = qs[k].z - p.z;
-            const float d2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, e2)));
-            bool take = v < n;
-            if (!take) take = __all(d2 >= dql);
-            if (take) {
-                const float ri = __builtin_amdgcn_rsqf(d2);
-                const float F = qs[k].w * (ri * ri * ri);
-                if (KAHAN) {
-                    float y, t;
-                    y = dx * F - kx; t = ax + y; kx = (t - ax) - y; ax = t;
-                    y = dy * F - ky; t = ay + y; ky = (t - ay) - y; ay = t;
-                    y = dz * F - kz; t = az + y; kz = (t - az) - y; az = t;
-                } else {
-                    ax = fmaf(dx, F, ax);
-                    ay = fmaf(dy, F, ay);
-                    az = fmaf(dz, F, az);
+        for (int c = 0; c < (PAIR ? 2 : 1); ++c) {
+            if (c >= cells) break;
+            const int lvl = lv[c];
+            const float dql = dqs[lvl];
+            const int vs[8] = {sv[c][0], sv[c][1], sv[c][2], sv[c][3], sv[c][4], sv[c][5], sv[c][6], sv[c][7]};
+            float4 qs[8];
+            qs[0] = make_float4(q0[c][0], q0[c][1], q0[c][2], q0[c][3]);
+            qs[1] = make_float4(q0[c][4], q0[c][5], q0[c][6], q0[c][7]);
+            qs[2] = make_float4(q0[c][8], q0[c][9], q0[c][10], q0[c][11]);
+            qs[3] = make_float4(q0[c][12], q0[c][13], q0[c][14], q0[c][15]);
+            qs[4] = make_float4(q1[c][0], q1[c][1], q1[c][2], q1[c][3]);
+            qs[5] = make_float4(q1[c][4], q1[c][5], q1[c][6], q1[c][7]);
+            qs[6] = make_float4(q1[c][8], q1[c][9], q1[c][10], q1[c][11]);
+            qs[7] = make_float4(q1[c][12], q1[c][13], q1[c][14], q1[c][15]);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int v = vs[k];
+                if (v < 0) continue;
+                const float dx = qs[k].x - p.x, dy = qs[k].y - p.y, dz = qs[k].z - p.z;
+                const float d2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, e2)));
+                bool take = v < n;
+                if (!take) take = __all(d2 >= dql);
+                if (take) {
+                    const float ri = __builtin_amdgcn_rsqf(d2);
+                    const float F = qs[k].w * (ri * ri * ri);
+                    if (KAHAN) {
+                        float y, t;
+                        y = dx * F - kx; t = ax + y; kx = (t - ax) - y; ax = t;
+                        y = dy * F - ky; t = ay + y; ky = (t - ay) - y; ay = t;
+                        y = dz * F - kz; t = az + y; kz = (t - az) - y; az = t;
+                    } else {
+                        ax = fmaf(dx, F, ax);
+                        ay = fmaf(dy, F, ay);
+                        az = fmaf(dz, F, az);
+                    }
+                    ++nint;
+                } else if (sp < kS) {  // the bounds above (one-cell walk: 7 * 64 + 8) keep sp below kS
+                    sk[sp] = ((v - n) << 7) | (lvl + 1);
+                    ++sp;
                 }
-                ++nint;
-            } else if (sp < kStack) {  // depth <= 64 keeps sp below 7*64+8
-                sk[sp] = ((v - n) << 7) | (lvl + 1);
-                ++sp;
             }
         }
     }
@@ -1174,13 +1220,14 @@ void launch_sort(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const
 void launch_force_order(hipStream_t s, const BlkWork* in, int len, int* keys, BlkWork* out) {
     if (len > 0 && len <= kOrderMax) NB_LAUNCH(k_force_order, 1, 1024, s, in, len, keys, out);
 }
-void launch_force_bh(hipStream_t s, int nblk, bool kahan, const SysDesc* sd, SysState* st,
+void launch_force_bh(hipStream_t s, int nblk, bool kahan, bool pair, const SysDesc* sd, SysState* st,
                      const BlkWork* work, const float4* node4, const int* child, const float4* cdata,
                      const int* order, float4* acc4, int* sorder) {
-    if (kahan)
-        NB_LAUNCH(k_force_bh<true>, nblk, 64 * kForceWaves, s, sd, st, work, node4, child, cdata, order, acc4, sorder);
-    else
-        NB_LAUNCH(k_force_bh<false>, nblk, 64 * kForceWaves, s, sd, st, work, node4, child, cdata, order, acc4, sorder);
+#define NB_FBH(K, P)                                                                                          \
+    if (kahan == K && pair == P)                                                                              \
+    NB_LAUNCH((k_force_bh<K, P>), nblk, 64 * kForceWaves, s, sd, st, work, node4, child, cdata, order, acc4, sorder)
+    NB_FBH(false, false); NB_FBH(true, false); NB_FBH(false, true); NB_FBH(true, true);
+#undef NB_FBH
 }
 void launch_range_table(hipStream_t s, BlkWork* out, int nblk, int sys, int start, int stop, int lo, int hi,
                         int per) {
