@@ -79,11 +79,13 @@ struct SysDesc {
 struct SysState {
     float box[6];
     int status;
-    int ncells;
     int max_depth;
+    // ncells and bq are the two halves of one 8-byte-aligned word: the build allocates
+    // cells and claims queued bodies with a single 64-bit atomic add
+    int ncells;
+    int bq;         // next entry of the build's last-phase body queue
     int ticket;     // bounds last-block ticket (self-resetting)
     int step;
-    int bq;         // next entry of the build's last-phase body queue
     float dq[kDqLen];
     int order_ok;   // 1: sorder holds a permutation of the bodies (the last full force pass's order)
     int pad1[2];

@@ -331,10 +331,24 @@ __global__ __launch_bounds__(BLK) void k_build(const SysDesc* __restrict__ sd, S
         if (threadIdx.x == 0) {
             int t = 0, u = 0;
             for (int q = 0; q < kWaves; ++q) t += s_wsum[q];
-            s_base = t > 0 ? atomicAdd(&S->ncells, t) : 0;
-            if (queued) {
+            if (queued)
                 for (int q = 0; q < kWaves; ++q) u += s_widle[q];
-                s_qbase = u > 0 && !drained && !ld_c(&S->status) ? atomicAdd(&S->bq, u) : nq;
+            // the cell allocation and the queue claim in one 64-bit atomic add on the
+            // adjacent counters (ncells low, bq high; neither can carry into the other),
+            // so the round pays one atomic round trip
+            const bool claim = queued && u > 0 && !drained;
+            int base = 0, qb = nq;
+            if (t > 0 || claim) {
+                const unsigned long long add =
+                    (unsigned long long)(unsigned)t | ((unsigned long long)(unsigned)(claim ? u : 0) << 32);
+                const unsigned long long old =
+                    atomicAdd(reinterpret_cast<unsigned long long*>(&S->ncells), add);
+                base = (int)(unsigned)old;
+                if (claim) qb = (int)(unsigned)(old >> 32);
+            }
+            s_base = base;
+            if (queued) {
+                s_qbase = qb;
                 drained = drained || s_qbase >= nq;
             }
         }
