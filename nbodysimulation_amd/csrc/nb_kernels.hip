@@ -617,7 +617,7 @@ constexpr int kStack = 8 * kDqLen;
 constexpr int kStackPair = 17 * kDqLen;
 // Which systems walk in pairs is a property of the system alone (its rule set), so that
 // a system's results do not depend on the batch it is in: REFERENCE-rule passes are bound
-// by the scalar-load latency of their longest walks and walk in pairs (1M: 2.69 -> 2.33 ms);
+// by the scalar loads their waves have in flight and walk in pairs (1M: 2.69 -> 2.33 ms);
 // STANDARD passes are bound by VALU issue and keep the one-cell walk, in a kernel instance
 // of their own (compiled into one kernel with the paired walk, the register and LDS
 // footprint of the pairs slowed them 1.50 -> 1.76 ms).  A launch of one instance skips the
