@@ -408,6 +408,9 @@ __global__ __launch_bounds__(BLK) void k_build(const SysDesc* __restrict__ sd, S
     }
 }
 
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
 // ---------------------------------------------------------------------------
 // summarize: bottom-up mass, centre of mass and body count (saturate_tree.cl:6-146)
 // without spin-waits.  Each cell ends with its children's (x, y, z, m) in one
@@ -446,34 +449,44 @@ __global__ __launch_bounds__(256) void k_summarize(const SysDesc* __restrict__ s
     int* SO = slotoff + 8 * (size_t)d.cell_off;
     int c = ci;
     for (;;) {
-        // the cell's slots, child block and per-slot counts in one batch of loads (coherent:
-        // the entries of child cells come from other lanes of this kernel)
+        // the cell's slots (32 B), child block (128 B) and per-slot counts (32 B) in one batch
+        // of twelve 16-byte loads, coherent at agent scope (sc1, as the compiler's agent-scope
+        // atomic loads: the entries of child cells come from other lanes of this kernel), with
+        // one wait inside the block, so no load is in flight when it ends
         int vs[8];
         float4 q[8];
         int sc[8];
         {
-            const unsigned long long* s8 = reinterpret_cast<const unsigned long long*>(C + 8 * c);
-            const unsigned long long* d8 = reinterpret_cast<const unsigned long long*>(CD + 8 * c);
-            const unsigned long long* o8 = reinterpret_cast<const unsigned long long*>(SO + 8 * c);
-            unsigned long long sv[4], dv[16], ov[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) sv[k] = ld_c64(s8 + k);
-#pragma unroll
-            for (int k = 0; k < 16; ++k) dv[k] = ld_c64(d8 + k);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) ov[k] = ld_c64(o8 + k);
+            v4i s0, s1, o0, o1;
+            v4f d0, d1, d2, d3, d4, d5, d6, d7;
+            asm volatile(
+                "global_load_dwordx4 %0, %12, off sc1\n\t"
+                "global_load_dwordx4 %1, %12, off offset:16 sc1\n\t"
+                "global_load_dwordx4 %2, %13, off sc1\n\t"
+                "global_load_dwordx4 %3, %13, off offset:16 sc1\n\t"
+                "global_load_dwordx4 %4, %13, off offset:32 sc1\n\t"
+                "global_load_dwordx4 %5, %13, off offset:48 sc1\n\t"
+                "global_load_dwordx4 %6, %13, off offset:64 sc1\n\t"
+                "global_load_dwordx4 %7, %13, off offset:80 sc1\n\t"
+                "global_load_dwordx4 %8, %13, off offset:96 sc1\n\t"
+                "global_load_dwordx4 %9, %13, off offset:112 sc1\n\t"
+                "global_load_dwordx4 %10, %14, off sc1\n\t"
+                "global_load_dwordx4 %11, %14, off offset:16 sc1\n\t"
+                "s_waitcnt vmcnt(0)"
+                : "=&v"(s0), "=&v"(s1), "=&v"(d0), "=&v"(d1), "=&v"(d2), "=&v"(d3), "=&v"(d4), "=&v"(d5),
+                  "=&v"(d6), "=&v"(d7), "=&v"(o0), "=&v"(o1)
+                : "v"(C + 8 * c), "v"(CD + 8 * c), "v"(SO + 8 * c)
+                : "memory");
+            const v4f dd[8] = {d0, d1, d2, d3, d4, d5, d6, d7};
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                vs[2 * k] = (int)(unsigned)sv[k];
-                vs[2 * k + 1] = (int)(unsigned)(sv[k] >> 32);
-                sc[2 * k] = (int)(unsigned)ov[k];
-                sc[2 * k + 1] = (int)(unsigned)(ov[k] >> 32);
+                vs[k] = s0[k];
+                vs[4 + k] = s1[k];
+                sc[k] = o0[k];
+                sc[4 + k] = o1[k];
             }
 #pragma unroll
-            for (int k = 0; k < 8; ++k)
-                q[k] = make_float4(__uint_as_float((unsigned)dv[2 * k]), __uint_as_float((unsigned)(dv[2 * k] >> 32)),
-                                   __uint_as_float((unsigned)dv[2 * k + 1]),
-                                   __uint_as_float((unsigned)(dv[2 * k + 1] >> 32)));
+            for (int k = 0; k < 8; ++k) q[k] = make_float4(dd[k][0], dd[k][1], dd[k][2], dd[k][3]);
         }
         float m = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
         int cnt = 0;
@@ -500,12 +513,13 @@ __global__ __launch_bounds__(256) void k_summarize(const SysDesc* __restrict__ s
             const int pk = cparent[d.cell_off + c];
             p = pk >> 3;
             const int ps = pk & 7;
-            float* f = reinterpret_cast<float*>(CD + 8 * p + ps);
-            st_cf(f, com.x);
-            st_cf(f + 1, com.y);
-            st_cf(f + 2, com.z);
-            st_cf(f + 3, com.w);
-            st_c(SO + 8 * p + ps, cnt);
+            // (COM, mass) in one 16-byte store and the count, both coherent (sc1) like the loads
+            const v4f cv = {com.x, com.y, com.z, com.w};
+            asm volatile("global_store_dwordx4 %0, %1, off sc1\n\t"
+                         "global_store_dword %2, %3, off sc1"
+                         :
+                         : "v"(CD + 8 * p + ps), "v"(cv), "v"(SO + 8 * p + ps), "v"(cnt)
+                         : "memory");
             // the coherent stores into the parent's blocks complete before its counter
             // drops, so the lane that takes the counter to zero reads them (sc1 loads)
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
