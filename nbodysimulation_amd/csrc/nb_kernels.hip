@@ -448,45 +448,55 @@ __global__ __launch_bounds__(256) void k_summarize(const SysDesc* __restrict__ s
     float4* CD = cdata + 8 * (size_t)d.cell_off;
     int* SO = slotoff + 8 * (size_t)d.cell_off;
     int c = ci;
-    for (;;) {
-        // the cell's slots (32 B), child block (128 B) and per-slot counts (32 B) in one batch
-        // of twelve 16-byte loads, coherent at agent scope (sc1, as the compiler's agent-scope
-        // atomic loads: the entries of child cells come from other lanes of this kernel), with
-        // one wait inside the block, so no load is in flight when it ends
+    for (bool first = true;; first = false) {
+        // The cell's slots and parent link are final since the build (ordinary cached loads).
+        // A lane's first cell has only bodies below it, whose child-block entries the build
+        // wrote: cached loads, and no counts (a body counts 1).  A cell it goes on with also
+        // holds entries that child cells wrote in this kernel: its child block (128 B) and
+        // per-slot counts (32 B) come in ten 16-byte loads, coherent at agent scope (sc1, as
+        // the compiler's agent-scope atomic loads), with one wait inside the block, so no load
+        // is in flight when it ends.
         int vs[8];
         float4 q[8];
-        int sc[8];
+        int sc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        int pk;
         {
-            v4i s0, s1, o0, o1;
-            v4f d0, d1, d2, d3, d4, d5, d6, d7;
-            asm volatile(
-                "global_load_dwordx4 %0, %12, off sc1\n\t"
-                "global_load_dwordx4 %1, %12, off offset:16 sc1\n\t"
-                "global_load_dwordx4 %2, %13, off sc1\n\t"
-                "global_load_dwordx4 %3, %13, off offset:16 sc1\n\t"
-                "global_load_dwordx4 %4, %13, off offset:32 sc1\n\t"
-                "global_load_dwordx4 %5, %13, off offset:48 sc1\n\t"
-                "global_load_dwordx4 %6, %13, off offset:64 sc1\n\t"
-                "global_load_dwordx4 %7, %13, off offset:80 sc1\n\t"
-                "global_load_dwordx4 %8, %13, off offset:96 sc1\n\t"
-                "global_load_dwordx4 %9, %13, off offset:112 sc1\n\t"
-                "global_load_dwordx4 %10, %14, off sc1\n\t"
-                "global_load_dwordx4 %11, %14, off offset:16 sc1\n\t"
-                "s_waitcnt vmcnt(0)"
-                : "=&v"(s0), "=&v"(s1), "=&v"(d0), "=&v"(d1), "=&v"(d2), "=&v"(d3), "=&v"(d4), "=&v"(d5),
-                  "=&v"(d6), "=&v"(d7), "=&v"(o0), "=&v"(o1)
-                : "v"(C + 8 * c), "v"(CD + 8 * c), "v"(SO + 8 * c)
-                : "memory");
-            const v4f dd[8] = {d0, d1, d2, d3, d4, d5, d6, d7};
+            const int4* c4 = reinterpret_cast<const int4*>(C + 8 * c);
+            const int4 s0 = c4[0], s1 = c4[1];
+            pk = c != 0 ? cparent[d.cell_off + c] : 0;
+            vs[0] = s0.x; vs[1] = s0.y; vs[2] = s0.z; vs[3] = s0.w;
+            vs[4] = s1.x; vs[5] = s1.y; vs[6] = s1.z; vs[7] = s1.w;
+            if (first) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                vs[k] = s0[k];
-                vs[4 + k] = s1[k];
-                sc[k] = o0[k];
-                sc[4 + k] = o1[k];
+                for (int k = 0; k < 8; ++k) q[k] = CD[8 * c + k];
+            } else {
+                v4i o0, o1;
+                v4f d0, d1, d2, d3, d4, d5, d6, d7;
+                asm volatile(
+                    "global_load_dwordx4 %0, %10, off sc1\n\t"
+                    "global_load_dwordx4 %1, %10, off offset:16 sc1\n\t"
+                    "global_load_dwordx4 %2, %10, off offset:32 sc1\n\t"
+                    "global_load_dwordx4 %3, %10, off offset:48 sc1\n\t"
+                    "global_load_dwordx4 %4, %10, off offset:64 sc1\n\t"
+                    "global_load_dwordx4 %5, %10, off offset:80 sc1\n\t"
+                    "global_load_dwordx4 %6, %10, off offset:96 sc1\n\t"
+                    "global_load_dwordx4 %7, %10, off offset:112 sc1\n\t"
+                    "global_load_dwordx4 %8, %11, off sc1\n\t"
+                    "global_load_dwordx4 %9, %11, off offset:16 sc1\n\t"
+                    "s_waitcnt vmcnt(0)"
+                    : "=&v"(d0), "=&v"(d1), "=&v"(d2), "=&v"(d3), "=&v"(d4), "=&v"(d5), "=&v"(d6), "=&v"(d7),
+                      "=&v"(o0), "=&v"(o1)
+                    : "v"(CD + 8 * c), "v"(SO + 8 * c)
+                    : "memory");
+                const v4f dd[8] = {d0, d1, d2, d3, d4, d5, d6, d7};
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    sc[k] = o0[k];
+                    sc[4 + k] = o1[k];
+                }
+#pragma unroll
+                for (int k = 0; k < 8; ++k) q[k] = make_float4(dd[k][0], dd[k][1], dd[k][2], dd[k][3]);
             }
-#pragma unroll
-            for (int k = 0; k < 8; ++k) q[k] = make_float4(dd[k][0], dd[k][1], dd[k][2], dd[k][3]);
         }
         float m = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
         int cnt = 0;
@@ -510,7 +520,6 @@ __global__ __launch_bounds__(256) void k_summarize(const SysDesc* __restrict__ s
         bool more = false;
         int p = 0;
         if (c != 0) {
-            const int pk = cparent[d.cell_off + c];
             p = pk >> 3;
             const int ps = pk & 7;
             // (COM, mass) in one 16-byte store and the count, both coherent (sc1) like the loads
