@@ -193,6 +193,9 @@ nb_status Engine::activate(const std::vector<Slot>& slots) {
     long long bf_waves2 = 0;  // brute-force waves of the batch at two targets per lane
     any_bh = any_bf = any_collision = false;
     any_bh_ref = any_bh_std = false;
+    long long bh_bodies = 0;  // the batch's Barnes-Hut bodies: the queued build phase shares its budget by them
+    for (const Slot& sl : slots)
+        if (sl.spec.algo == NB_BARNES_HUT && sl.spec.n > 0) bh_bodies += sl.spec.n;
     for (const Slot& sl : slots) {
         if (sl.slot < 0 || sl.slot >= capc.systems || seen[sl.slot]) {
             set_error("slot index out of range or used twice");
@@ -246,8 +249,16 @@ nb_status Engine::activate(const std::vector<Slot>& slots) {
                 long long items = (d.n + kPhaseStride[ph] - 1) / kPhaseStride[ph];
                 // the queued last phase takes its bodies from the system's queue, so its
                 // workgroup count is free: 5/8 of the CUs' worth of workgroups contend less
-                // for the same slots than one per 1024 bodies (1M: build 0.42 -> 0.35 ms)
-                if (kPhaseStride[ph] == 1) items = std::min<long long>(items, (long long)std::max(1, cus * 5 / 8) * blk);
+                // for the same slots than one per 1024 bodies (1M: build 0.42 -> 0.35 ms).  A
+                // batch shares twice the CU count among its systems by their bodies, at least
+                // one workgroup each and at most the single system's 5/8 (512 systems: 91.6 ->
+                // 109 ensemble steps/s against one workgroup per 1024 bodies of every system;
+                // 107 with 1x, 110 with 4x the CU count)
+                if (kPhaseStride[ph] == 1) {
+                    const long long cap = std::max(1, cus * 5 / 8), budget = 2LL * cus;
+                    const long long share = std::max<long long>(1, (budget * d.n + bh_bodies - 1) / bh_bodies);
+                    items = std::min<long long>(items, std::min(share, cap) * blk);
+                }
                 add_ranges(t[T_PHASE0 + ph], s, (int)items, blk);
             }
         }
