@@ -92,6 +92,8 @@ def parse_args():
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--no-shares", action="store_true", help="bh1m: skip the per-GPU target-share timing")
     ap.add_argument("--no-companion", action="store_true", help="bh1m: skip the timed run under the other rule set")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="ensemble: skip the one-system-per-launch comparison legs (time the batched step only)")
     ap.add_argument("--requests", type=int, default=2000, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=0.0,
                     help="serving: Poisson arrival rate, requests/s (0: --load of the measured capacity)")
@@ -556,8 +558,9 @@ def bench_ensemble(args):
             hip.hipStreamDestroy(__import__("ctypes").c_void_p(stream))
         return t
 
-    t_plain_shared = plain(True)
-    t_plain_own = plain(False)
+    # the comparison legs (one system per launch); --no-plain skips them (profiling the batched step)
+    t_plain_shared = None if args.no_plain else plain(True)
+    t_plain_own = None if args.no_plain else plain(False)
     err = ensemble_force_error(specs, ics)
     log(f"bench: ensemble force error vs FP64 median {err['median']:.3e} p99 {err['p99']:.3e}")
     bodies = int(sizes.sum())
@@ -570,7 +573,7 @@ def bench_ensemble(args):
                    "crossover_n": cross, "barnes_hut_systems": int(sum(a == L.BARNES_HUT for a in algos)),
                    "hbm_bytes": hbm, "device": info},
         "bodies_steps_per_s": bodies * steps / t_batched,
-        "plain_one_system_per_launch": {
+        "plain_one_system_per_launch": None if args.no_plain else {
             "shared_stream_steps_per_s": steps / t_plain_shared,
             "per_system_streams_steps_per_s": steps / t_plain_own,
             "speedup_vs_shared_stream": t_plain_shared / t_batched,
@@ -578,8 +581,11 @@ def bench_ensemble(args):
         "autotune": tune,
         "force_error": err,
     }
-    log(f"bench: batched {steps / t_batched:.2f} ensemble steps/s; plain shared stream {steps / t_plain_shared:.2f}, "
-        f"own streams {steps / t_plain_own:.2f}")
+    if args.no_plain:
+        log(f"bench: batched {steps / t_batched:.2f} ensemble steps/s")
+    else:
+        log(f"bench: batched {steps / t_batched:.2f} ensemble steps/s; plain shared stream "
+            f"{steps / t_plain_shared:.2f}, own streams {steps / t_plain_own:.2f}")
     print(json.dumps(line), flush=True)
 
 
