@@ -24,6 +24,8 @@ void launch_bounds(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, con
 void launch_tree_clear(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
                        int* child, int* cpend);
 void launch_tree_init(hipStream_t s, int nsys, const int* slots, SysState* st);  // slots NULL: 0..nsys-1
+void launch_build_top(hipStream_t s, int nsys, const int* slots, const SysDesc* sd, SysState* st,
+                      const float4* node4, int* child, int* cparent, int* cpend, float4* cdata, int stride);
 void launch_build(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
                   const float4* node4, int* child, int* cparent, int* cpend, float4* cdata, int stride,
                   int exclude);

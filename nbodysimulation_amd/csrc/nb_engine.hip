@@ -350,6 +350,7 @@ void Engine::tree_reset() {
 }
 void Engine::build() {
     TraceRange tr("nb:build");
+    launch_build_top(stream, nactive, d_slots, d_desc, d_state, node4, child, cparent, cpend, cdata, kPhaseStride[0]);
     for (int ph = 0; ph < kBuildPhases; ++ph)
         launch_build(stream, tab_len[T_PHASE0 + ph], d_desc, d_state, tab[T_PHASE0 + ph], node4, child, cparent,
                      cpend, cdata, kPhaseStride[ph], ph == 0 ? 0 : kPhaseStride[ph - 1]);

@@ -88,7 +88,8 @@ struct SysState {
     int step;
     float dq[kDqLen];
     int order_ok;   // 1: sorder holds a permutation of the bodies (the last full force pass's order)
-    int pad1[2];
+    int top;        // 1: this step's first insertion phase was built in LDS (k_build_top)
+    int pad1;
 };
 
 // One workgroup's share of one system: items [begin, end).  `trim` is only set

@@ -160,6 +160,7 @@ __global__ void k_tree_init(int nsys, const int* __restrict__ slots, SysState* s
     st[s].ncells = 1;
     st[s].max_depth = 1;  // build_tree.cl:28
     st[s].bq = 0;
+    st[s].top = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -205,6 +206,7 @@ __global__ __launch_bounds__(BLK) void k_build(const SysDesc* __restrict__ sd, S
     const SysDesc d = sd[w.sys];
     SysState* S = st + w.sys;
     if (S->status) return;
+    if (exclude == 0 && S->top) return;  // the first phase was built in LDS (k_build_top)
     const int n = d.n;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -410,6 +412,113 @@ __global__ __launch_bounds__(BLK) void k_build(const SysDesc* __restrict__ sd, S
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// The first insertion phase (every stride-th body) in LDS: one workgroup per system, one
+// lane per body of the phase, builds that phase's tree level by level, then writes the
+// cells (slots, parent links, counts of child cells), the cell count and the depth, and
+// sets S->top so that k_build skips the phase.  Per level, each body not yet placed claims
+// the slot of its current cell with a CAS and a loser marks the slot contested; after a
+// barrier an uncontested body is placed (its child-block entry goes to cdata), and the
+// body holding a contested slot allocates a new cell there; after another barrier the
+// contested bodies move into it.  So a cell exists exactly for every path prefix that two
+// or more of the bodies share, which is the topology k_build's insertion gives.  Cell
+// numbers are those of the LDS tree.  More than kTopBodies bodies, or more than kTopCells
+// cells, writes nothing: S->top stays 0 and k_build runs the phase in global memory.
+
+constexpr int kTopCells = 1280;  // 50 KiB of LDS: 8 slots, a parent link and contested bits per cell
+constexpr int kTopBodies = 512;  // 8 KiB of LDS: the phase's bodies
+
+__global__ __launch_bounds__(kTopBodies) void k_build_top(int nsys, const int* __restrict__ slots,
+                                                          const SysDesc* __restrict__ sd, SysState* st,
+                                                          const float4* __restrict__ node4, int* child, int* cparent,
+                                                          int* cpend, float4* cdata, int stride) {
+    __shared__ int sC[8 * kTopCells];
+    __shared__ int sPar[kTopCells];
+    __shared__ int sCont[kTopCells];  // bit k: slot k was claimed by more than one body this level
+    __shared__ float4 sP[kTopBodies];
+    __shared__ int sN, sFail, sDepth;
+    const int sys = slots ? slots[blockIdx.x] : blockIdx.x;
+    const SysDesc d = sd[sys];
+    SysState* S = st + sys;
+    const int n = d.n;
+    const int nb = (n + stride - 1) / stride;  // bodies j * stride, j < nb
+    if (blockIdx.x >= nsys || d.algo != 0 || n == 0 || nb > kTopBodies || S->status) return;
+    for (int i = threadIdx.x; i < 8 * kTopCells; i += blockDim.x) sC[i] = kEmpty;
+    for (int i = threadIdx.x; i < kTopCells; i += blockDim.x) sCont[i] = 0;
+    const int j = threadIdx.x;
+    if (j < nb) sP[j] = node4[d.node_off + j * stride];
+    if (threadIdx.x == 0) {
+        sN = 1;
+        sFail = 0;
+        sDepth = 1;
+    }
+    float4* CD = cdata + 8 * (size_t)d.cell_off;
+    float lx, ly, lz, hx, hy, hz;
+    root_box(d.rules, S->box, lx, ly, lz, hx, hy, hz);
+    const int cap = min(kTopCells, d.cap);
+    const int b = j * stride;
+    bool active = j < nb;
+    int node = 0, lvl = 0, depth = 1;
+    __syncthreads();
+    const float4 p = active ? sP[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (;;) {
+        int slot = 0, dg = 0;
+        if (active) {  // claim the slot of this level
+            dg = level_digit(d.rules, lvl, p.x, p.y, p.z, lx, ly, lz, hx, hy, hz);
+            slot = 8 * node + dg;
+            if (atomicCAS(sC + slot, kEmpty, b) != kEmpty) atomicOr(sCont + node, 1 << dg);
+        }
+        __syncthreads();
+        if (active) {
+            if (!((sCont[node] >> dg) & 1)) {  // alone in its slot: placed
+                CD[slot] = p;
+                depth = max(depth, lvl);
+                active = false;
+            } else if (sC[slot] == b) {  // the holder of a contested slot opens a cell there
+                if (lvl + 1 > kMaxLevel) {
+                    atomicOr(&S->status, kStDepthOverflow);
+                    sFail = 1;
+                } else {
+                    const int c = atomicAdd(&sN, 1);
+                    if (c >= cap) {
+                        sFail = 1;
+                    } else {
+                        sPar[c] = (node << 3) | dg;
+                        sC[slot] = n + c;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (sFail) break;  // written only between the two barriers above: every lane reads the same
+        if (active) {      // into the new cell, one level down
+            node = sC[slot] - n;
+            ++lvl;
+        }
+        if (!__syncthreads_or(active)) break;
+    }
+    depth = wave_max_i(depth);
+    if ((threadIdx.x & 63) == 0) atomicMax(&sDepth, depth);
+    __syncthreads();
+    if (sFail) return;
+    const int nc = sN;
+    int* C = child + 8 * (size_t)d.cell_off;
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+        const int4 a = make_int4(sC[8 * c], sC[8 * c + 1], sC[8 * c + 2], sC[8 * c + 3]);
+        const int4 e = make_int4(sC[8 * c + 4], sC[8 * c + 5], sC[8 * c + 6], sC[8 * c + 7]);
+        reinterpret_cast<int4*>(C + 8 * c)[0] = a;
+        reinterpret_cast<int4*>(C + 8 * c)[1] = e;
+        cpend[d.cell_off + c] = (a.x >= n) + (a.y >= n) + (a.z >= n) + (a.w >= n) + (e.x >= n) + (e.y >= n) +
+                                (e.z >= n) + (e.w >= n);
+        if (c > 0) cparent[d.cell_off + c] = sPar[c];
+    }
+    if (threadIdx.x == 0) {
+        S->ncells = nc;
+        S->max_depth = sDepth;
+        S->top = 1;
+    }
+}
 
 // ---------------------------------------------------------------------------
 // summarize: bottom-up mass, centre of mass and body count (saturate_tree.cl:6-146)
@@ -1230,6 +1339,11 @@ void launch_tree_clear(hipStream_t s, int nblk, const SysDesc* sd, const SysStat
 void launch_tree_init(hipStream_t s, int nsys, const int* slots, SysState* st) {
     if (nsys <= 0) return;
     NB_LAUNCH(k_tree_init, (nsys + 255) / 256, 256, s, nsys, slots, st);
+}
+void launch_build_top(hipStream_t s, int nsys, const int* slots, const SysDesc* sd, SysState* st,
+                      const float4* node4, int* child, int* cparent, int* cpend, float4* cdata, int stride) {
+    if (nsys <= 0) return;
+    NB_LAUNCH(k_build_top, nsys, kTopBodies, s, nsys, slots, sd, st, node4, child, cparent, cpend, cdata, stride);
 }
 void launch_build(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
                   const float4* node4, int* child, int* cparent, int* cpend, float4* cdata, int stride,

@@ -345,6 +345,36 @@ def test_full_size_tree_exact(orc, gpu, rules):
         s.close()
 
 
+TOP_CELLS = 1280  # cells of the LDS tree that the first insertion phase (every 4096th body) may use
+
+
+@pytest.mark.parametrize("rules", [0, 1])
+def test_first_phase_overflow_keeps_the_tree_exact(orc, gpu, rules):
+    """The first insertion phase (every 4096th body) is built in LDS unless its tree needs more
+    than TOP_CELLS cells; then the phase runs in global memory.  Here every odd 4096th body of a
+    1M Plummer system sits a few ulps from the even one before it, so the phase's bodies need
+    thousands of chain cells (counted by the oracle on the CPU below): the tree must still equal
+    the oracle's bit for bit."""
+    n = 1 << 20
+    say(f"case first-phase overflow rules={rules}")
+    pos, vel, mass = ic.plummer(n, 1)
+    pos = pos.copy()
+    even = np.arange(0, n, 2 * 4096)
+    odd = even + 4096
+    pos[odd] = np.nextafter(np.nextafter(pos[even], np.float32(np.inf)), np.float32(np.inf))
+    box = np.concatenate([pos.min(0), pos.max(0)]).astype(np.float32)
+    first = orc.Tree(pos[::4096], mass[::4096], rules, box=box, cap=1 << 20)
+    assert first.status == orc.OK and first.ncells > TOP_CELLS  # the LDS tree overflows
+    s, gbox = gpu_tree(pos, vel, mass, rules, theta=0.5, eps=0.01, G=1.0)
+    try:
+        assert gbox.tobytes() == box.tobytes()
+        t = orc.Tree(pos, mass, rules, cap=s.tree_info()["capacity"])
+        assert t.status == orc.OK
+        compare_tree(s, t, n)
+    finally:
+        s.close()
+
+
 def test_largest_served_system_tree_exact(orc, gpu):
     """N = 2,097,152 (the largest system the serving workload admits), STANDARD rules, with
     the server's pool sizing (2N + 1024 cells): bit-exact tree and sort against the oracle."""
