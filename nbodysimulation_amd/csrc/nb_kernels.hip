@@ -438,12 +438,13 @@ __global__ __launch_bounds__(kTopBodies) void k_build_top(int nsys, const int* _
     __shared__ int sCont[kTopCells];  // bit k: slot k was claimed by more than one body this level
     __shared__ float4 sP[kTopBodies];
     __shared__ int sN, sFail, sDepth;
+    if ((int)blockIdx.x >= nsys) return;
     const int sys = slots ? slots[blockIdx.x] : blockIdx.x;
     const SysDesc d = sd[sys];
     SysState* S = st + sys;
     const int n = d.n;
     const int nb = (n + stride - 1) / stride;  // bodies j * stride, j < nb
-    if (blockIdx.x >= nsys || d.algo != 0 || n == 0 || nb > kTopBodies || S->status) return;
+    if (d.algo != 0 || n == 0 || nb > kTopBodies || S->status) return;
     for (int i = threadIdx.x; i < 8 * kTopCells; i += blockDim.x) sC[i] = kEmpty;
     for (int i = threadIdx.x; i < kTopCells; i += blockDim.x) sCont[i] = 0;
     const int j = threadIdx.x;
