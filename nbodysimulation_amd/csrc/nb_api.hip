@@ -994,18 +994,7 @@ struct nb_ensemble {
     hipGraph_t graph_def = nullptr;
 };
 
-static void ens_enqueue_force(nb_ensemble* e) {
-    Engine& g = e->eng;
-    if (g.any_bh) {
-        g.bounds();
-        g.tree_reset();
-        g.build();
-        g.summarize();
-        g.sort();
-        g.force();
-    }
-    if (g.any_bf) g.brute();
-}
+static void ens_enqueue_force(nb_ensemble* e) { e->eng.force_all(); }
 
 static void ens_enqueue_step(nb_ensemble* e) {
     const bool lf = e->method == NB_LEAPFROG;

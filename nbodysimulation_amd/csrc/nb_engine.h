@@ -110,6 +110,11 @@ struct Engine {
     int cus = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    // a batch that holds both kinds of systems runs its brute-force launch on `side`, beside
+    // the Barnes-Hut stages on `stream` (forked and joined by the two events); created by
+    // activate for the first such batch
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
     Capacity capc;
     std::vector<SysDesc> hdesc;  // indexed by slot
@@ -161,7 +166,10 @@ struct Engine {
     void summarize();
     void sort();
     void force();
-    void brute();
+    void brute(hipStream_t on = nullptr);  // on: the stream to launch on (default: stream)
+    // the force pass of every resident system: the Barnes-Hut stages, and brute force beside
+    // them on `side` when the batch holds both kinds (joined back into `stream`)
+    void force_all();
     // Brute force with the integrator of `mode` (0 Euler, 1 closing kick) fused:
     // positions are read from `in` (node4 layout) and, for mode 0, the moved ones
     // written to `out`; velocities are updated in place.

@@ -31,6 +31,11 @@ static constexpr int kBoundsPerBlock = 2048;
 Engine::~Engine() { release(); }
 
 void Engine::release() {
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (side) (void)hipStreamDestroy(side);
+    ev_fork = ev_join = nullptr;
+    side = nullptr;
     if (arena) (void)hipFree(arena);
     arena = nullptr;
     if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -290,6 +295,11 @@ nb_status Engine::activate(const std::vector<Slot>& slots) {
     // longest-first dispatch only where a launch holds at most four waves per SIMD slot
     // (8 waves x 4 SIMDs per CU): with more, the last workgroups matter less than locality
     lpt = tab_len[T_GROUPS] <= kOrderMax && (long long)tab_len[T_GROUPS] * kForceWaves <= 4LL * 32 * cus;
+    if (any_bh && any_bf && !side) {
+        NB_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        NB_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        NB_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    }
     NB_HIP(hipMemcpyAsync(d_desc, hdesc.data(), sizeof(SysDesc) * capc.systems, hipMemcpyHostToDevice, stream));
     if (!ids.empty()) NB_HIP(hipMemcpyAsync(d_slots, ids.data(), sizeof(int) * ids.size(), hipMemcpyHostToDevice, stream));
     NB_HIP(hipStreamSynchronize(stream));
@@ -391,10 +401,39 @@ void Engine::brute_range(int sys, int lo, int hi) {
     launch_range_table(stream, tab[T_RANGE], nblk, sys, lo, hi, lo, hi, per);
     launch_brute(stream, nblk, law, kahan, bf_tb, d_desc, d_state, tab[T_RANGE], node4, acc4, vel4, nullptr, -1);
 }
-void Engine::brute() {
+void Engine::brute(hipStream_t on) {
     TraceRange tr("nb:brute_force");
-    launch_brute(stream, tab_len[T_BRUTE], law, kahan, bf_tb, d_desc, d_state, tab[T_BRUTE], node4, acc4, vel4,
-                 nullptr, -1);
+    launch_brute(on ? on : stream, tab_len[T_BRUTE], law, kahan, bf_tb, d_desc, d_state, tab[T_BRUTE], node4, acc4,
+                 vel4, nullptr, -1);
+}
+// The brute-force kernel reads and writes only brute-force systems' positions and
+// accelerations, which no Barnes-Hut stage touches, so in a mixed batch it runs beside the
+// latency-bound summarize and sort instead of after the force pass (forked after the build,
+// whose last phase fills every CU in a batch: 512-system ensemble 109.4-109.9 -> 111.0
+// steps/s; forked before the bounds, 110.6-110.8).  Errors of the fork and join surface
+// through hipGetLastError (or the end of a graph capture), like those of the launches.
+void Engine::force_all() {
+    const bool beside = any_bh && any_bf && side;
+    if (any_bh) {
+        bounds();
+        tree_reset();
+        build();
+    }
+    if (beside) {
+        (void)hipEventRecord(ev_fork, stream);
+        (void)hipStreamWaitEvent(side, ev_fork, 0);
+        brute(side);
+        (void)hipEventRecord(ev_join, side);
+    }
+    if (any_bh) {
+        summarize();
+        sort();
+        force();
+    }
+    if (beside)
+        (void)hipStreamWaitEvent(stream, ev_join, 0);
+    else if (any_bf)
+        brute();
 }
 void Engine::brute_step(int mode, const float4* in, float4* out) {
     TraceRange tr("nb:brute_force_step");

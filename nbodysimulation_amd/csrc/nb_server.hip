@@ -244,17 +244,7 @@ static nb_status download_request(nb_server* sv, Request& r) {
     return NB_OK;
 }
 
-static void enqueue_force(nb::Engine& g) {
-    if (g.any_bh) {
-        g.bounds();
-        g.tree_reset();
-        g.build();
-        g.summarize();
-        g.sort();
-        g.force();
-    }
-    if (g.any_bf) g.brute();
-}
+static void enqueue_force(nb::Engine& g) { g.force_all(); }
 
 extern "C" {
 
