@@ -1,0 +1,544 @@
+// nb_build.hip -- the bucketed insertion phases of the tree build (build_tree.cl:42-161
+// computes the same tree by lock-free insertion of every body from the root).
+//
+// After the sparse first phase the tree holds a skeleton T (final while a phase runs).
+// A later phase inserts its bodies in four kernels, with no contention between lanes:
+//   k_classify    each body of the phase descends T (read-only) to the slot where its
+//                 insertion would start, the bucket key 8*cell + slot, and takes a rank
+//                 among the bucket's bodies (one atomic add on the bucket's counter);
+//   k_frames      per bucket of T's slots: the bucket's size (its bodies, plus the body
+//                 already in the slot, if any); buckets of at most kFrame bodies are packed
+//                 into frames of at most kFrame bodies, each bucket contiguous in the frame
+//                 list; larger buckets are marked for the fallback;
+//   k_scatter     each body goes to its bucket's place in the frame list (or to the
+//                 fallback list);
+//   k_frame_build one workgroup per frame builds every bucket's subtree in LDS, level by level
+//                 (each body claims its slot by CAS, a loser marks the slot contested; an
+//                 uncontested body is placed, the holder of a contested slot opens a cell
+//                 there, the contested bodies move into it), so a cell exists exactly for
+//                 every path prefix that two or more of the bucket's bodies share: the
+//                 topology that insertion gives, whatever the order.  The workgroup then takes
+//                 one range of cell numbers (one atomic add), writes the cells, the bodies'
+//                 child-block entries and the link of each subtree into its slot of T.
+// Buckets above kFrame bodies, and frames whose subtrees do not fit the LDS cell pool or
+// reach the depth limit, go to k_build's queued insertion (list mode) afterwards.
+//
+// Scratch during the build (each array is written afresh by a later stage):
+//   sorted / forder   per body: bucket key and rank (the sort writes both permutations)
+// and of its own: the frame list (per entry the body's position; its id, bucket key and the
+// level of the bucket's cell), the frames (list begin, size), the fallback list, and per slot
+// of T the bucket's place in the frame list (kbase)
+//   slotoff           per slot of T: the bucket's body count, cleared by k_frames (summarize
+//                     writes the per-slot counts; cells are created with cleared counts)
+//   ccount            per cell of T that ends a descent: its level (summarize writes counts)
+#include <hip/hip_runtime.h>
+
+#include "nb_device.h"
+#include "nb_internal.h"
+
+namespace nb {
+
+constexpr int kFrameThreads = 128;             // one k_frame_build workgroup builds one frame
+constexpr int kFrame = 4 * kFrameThreads;      // bodies per frame: four per lane
+constexpr int kFrameCells = 384;               // cells of one frame's subtrees in LDS
+// Bodies packed per frame: the subtrees of STANDARD-rule systems take about 0.48 cells per
+// body, those of REFERENCE-rule systems (cells halved in y by the x comparison) about 0.72,
+// so that a frame's cells stay within kFrameCells (above it the frame goes to the fallback).
+__host__ __device__ constexpr int frame_budget(int rules) { return rules == 0 ? 448 : kFrame; }
+constexpr int kCellTag = 1 << 16;              // LDS slot value >= kCellTag: local cell (value - kCellTag)
+
+// The bodies of one insertion phase: b = bi * stride for the block's items bi, without the
+// multiples of `exclude` (bodies of earlier phases).  Returns -1 for an item outside it.
+// (32-bit arithmetic: the block tables keep bi * stride below n + stride.  Written with a
+// 64-bit remainder, hipcc for gfx950 dropped the value on the path where the remainder is
+// non-zero, so every strided item came out as -1.)
+__device__ __forceinline__ int phase_body(const BlkWork& w, int n, int stride, int exclude) {
+    const int bi = w.begin + (int)threadIdx.x;
+    const int b = bi * stride;
+    const bool in = bi < w.end && b < n && (exclude <= 0 || b % exclude != 0);
+    return in ? b : -1;
+}
+
+// The body of a phase's item i = w.begin + thread: a phase over every body (stride 1) takes
+// the bodies in the last force pass's order when there is one (as the sort does), so that
+// the lanes of a wave descend the same cells and fall into few buckets; a strided phase
+// takes every stride-th body.  -1: no body of the phase.
+__device__ __forceinline__ int phase_item(const BlkWork& w, const SysState* S, const int* sorder, int n, int stride,
+                                          int exclude) {
+    if (stride != 1 || !S->order_ok) return phase_body(w, n, stride, exclude);
+    const int i = w.begin + (int)threadIdx.x;
+    const int b = i < w.end && i < n ? min(max(sorder[i], 0), n - 1) : -1;
+    return b >= 0 && !(exclude > 0 && b % exclude == 0) ? b : -1;
+}
+
+// In the last force pass's order the lanes of a wave fall into few buckets: each bucket's
+// count then takes one atomic add per wave (ranks within the wave by ballot), not one per
+// body.  Key and rank are stored by the item's place in the enumeration (coalesced), and
+// k_scatter enumerates the same way.
+__global__ __launch_bounds__(256) void k_classify(const SysDesc* __restrict__ sd, SysState* st,
+                                                  const BlkWork* __restrict__ work, const float4* __restrict__ node4,
+                                                  const int* __restrict__ child, int* slotoff, int* ccount, int* bkey,
+                                                  int* brank, const int* __restrict__ sorder, int stride,
+                                                  int exclude) {
+    const BlkWork w = work[blockIdx.x];
+    const SysDesc d = sd[w.sys];
+    SysState* S = st + w.sys;
+    if (S->status || d.algo != 0) return;
+    // the phase's counters start at zero (the kernels that used them last have completed)
+    if (w.begin == 0 && threadIdx.x == 0) {
+        S->nframes = 0;
+        S->listpos = 0;
+        S->nfb = 0;
+        S->bq = 0;
+    }
+    const int n = d.n;
+    const int i = w.begin + (int)threadIdx.x;  // the phase's enumeration: bkey/brank are indexed by i
+    const int b = phase_item(w, S, sorder + d.body_off, n, stride, exclude);
+    if (b < 0) return;
+    const int* C = child + 8 * (size_t)d.cell_off;
+    const float4 p = node4[d.node_off + b];
+    float lx, ly, lz, hx, hy, hz;
+    root_box(d.rules, S->box, lx, ly, lz, hx, hy, hz);
+    int node = 0, lvl = 0, key = -1;
+    for (; lvl <= kMaxLevel; ++lvl) {
+        const int dg = level_digit(d.rules, lvl, p.x, p.y, p.z, lx, ly, lz, hx, hy, hz);
+        const int v = C[8 * node + dg];
+        if (v >= n && v - n < d.cap) {
+            node = v - n;
+            continue;
+        }
+        if (v >= kEmpty && v < n) key = 8 * node + dg;  // empty, or a body: no insertion runs, so no lock
+        break;
+    }
+    if (key < 0) {  // an inconsistent or too deep tree; cannot happen on a tree the build made
+        atomicOr(&S->status, kStStall);
+        return;
+    }
+    // in force order, one count per distinct bucket of the wave: the first lane of each key
+    // adds the lanes that share it, each lane's rank follows its place among them (in a
+    // strided phase the lanes' bodies are far apart and count one by one)
+    int* cnt = slotoff + 8 * (size_t)d.cell_off;
+    const int lane = threadIdx.x & 63;
+    int rank = 0;
+    if (!(stride == 1 && S->order_ok)) {
+        rank = atomicAdd(cnt + key, 1);
+    } else {
+        // groups of lanes by key (no memory traffic), then every group's first lane adds the
+        // group's size in one vector atomic: one round trip per wave
+        int leader = lane, gsize = 1, below = 0;
+        for (unsigned long long todo = __ballot(true); todo;) {
+            const int l0 = __ffsll((long long)todo) - 1;
+            const int lk = __shfl(key, l0);
+            const unsigned long long m = __ballot(key == lk);
+            if (key == lk) {
+                leader = l0;
+                gsize = __popcll(m);
+                below = __popcll(m & ((1ull << lane) - 1));
+            }
+            todo &= ~m;
+        }
+        int base = 0;
+        if (lane == leader) base = atomicAdd(cnt + key, gsize);
+        rank = __shfl(base, leader) + below;
+    }
+    brank[d.body_off + i] = rank;
+    bkey[d.body_off + i] = key;
+    ccount[d.cell_off + node] = lvl;  // the same value from every body that ends here
+}
+
+// One lane per slot of T (grid-stride: the block is w.begin of the system's w.end blocks and
+// takes 256 consecutive slots at a time).  Each wave packs its 64 slots' buckets greedily in
+// slot order (one lane), then the block claims its waves' frames and list space with one
+// 64-bit atomic add (frames low, list entries high): same-address atomics cost about as
+// much as a wave's whole packing, so there is one per block and round.
+__global__ __launch_bounds__(256) void k_frames(const SysDesc* __restrict__ sd, SysState* st,
+                                                const BlkWork* __restrict__ work, const float4* __restrict__ node4,
+                                                const int* __restrict__ child, int* slotoff,
+                                                const int* __restrict__ ccount, int* kbase, float4* lpos, int4* lmeta,
+                                                int2* bframe) {
+    __shared__ int s_sz[4][64], s_f[4][64], s_o[4][64], s_fo[4][64], s_fs[4][64];
+    __shared__ int s_nf[4], s_tot[4], s_fb, s_lb, s_ok;
+    const BlkWork w = work[blockIdx.x];
+    const SysDesc d = sd[w.sys];
+    SysState* S = st + w.sys;
+    if (S->status || d.algo != 0) return;
+    const int n = d.n;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nkeys = 8 * min(S->ncells, d.cap);
+    const int budget = frame_budget(d.rules);
+    const int* C = child + 8 * (size_t)d.cell_off;
+    int* cnt8 = slotoff + 8 * (size_t)d.cell_off;
+    int* kb = kbase + 8 * (size_t)d.cell_off;
+    float4* LP = lpos + d.body_off;
+    int4* LM = lmeta + d.body_off;
+    int2* frames = bframe + d.body_off;
+    for (int k0 = w.begin * 256; k0 < nkeys; k0 += w.end * 256) {  // uniform over the block
+        const int k = k0 + (int)threadIdx.x;
+        int cnt = 0, res = -1;
+        if (k < nkeys) {
+            cnt = cnt8[k];
+            if (cnt) {
+                cnt8[k] = 0;  // cleared for the next phase's counts
+                const int v = C[k];
+                res = (v >= 0 && v < n) ? v : -1;  // the body already in the slot joins the bucket
+            }
+        }
+        const int size = cnt ? cnt + (res >= 0 ? 1 : 0) : 0;
+        const bool big = size > budget;
+        const int sz = big ? 0 : size;
+        s_sz[wave][lane] = sz;
+        wave_sync();
+        if (lane == 0) {  // greedy packing of the wave's slots in slot order
+            int f = 0, used = 0, tot = 0;
+            for (int j = 0; j < 64; ++j) {
+                const int sj = s_sz[wave][j];
+                if (sj == 0) continue;
+                if (used + sj > budget) {
+                    s_fs[wave][f++] = used;
+                    used = 0;
+                }
+                s_f[wave][j] = f;
+                s_o[wave][j] = used;
+                used += sj;
+                tot += sj;
+            }
+            int nf = f;
+            if (used > 0) s_fs[wave][nf++] = used;
+            for (int q = 0, o = 0; q < nf; ++q) {
+                s_fo[wave][q] = o;
+                o += s_fs[wave][q];
+            }
+            s_nf[wave] = nf;
+            s_tot[wave] = tot;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {  // one claim for the block: frames low, list entries high
+            int nf = 0, tot = 0;
+            for (int q = 0; q < 4; ++q) {
+                nf += s_nf[q];
+                tot += s_tot[q];
+            }
+            unsigned long long old = 0;
+            if (nf > 0)
+                old = atomicAdd(reinterpret_cast<unsigned long long*>(&S->nframes),
+                                (unsigned long long)(unsigned)nf | ((unsigned long long)(unsigned)tot << 32));
+            s_fb = (int)(unsigned)old;
+            s_lb = (int)(unsigned)(old >> 32);
+            // frames <= buckets <= bodies of the phase; the list holds the phase's bodies and the
+            // bodies of earlier phases already in the buckets' slots, so at most n entries
+            s_ok = !(nf > 0 && (s_fb + nf > n || s_lb + tot > n));
+            if (!s_ok) atomicOr(&S->status, kStStall);
+        }
+        __syncthreads();
+        if (!s_ok) return;
+        int fb = s_fb, lb = s_lb;
+        for (int q = 0; q < wave; ++q) {
+            fb += s_nf[q];
+            lb += s_tot[q];
+        }
+        const int nf = s_nf[wave];
+        if (sz > 0) {
+            const int pos = lb + s_fo[wave][s_f[wave][lane]] + s_o[wave][lane];
+            if (res >= 0) {
+                LP[pos] = node4[d.node_off + res];
+                LM[pos] = make_int4(res, k, ccount[d.cell_off + (k >> 3)], 0);
+            }
+            kb[k] = pos + (res >= 0 ? 1 : 0);
+        } else if (big) {
+            kb[k] = -1;  // its bodies go to the fallback insertion; the body in the slot stays
+        }
+        if (lane < nf) frames[fb + lane] = make_int2(lb + s_fo[wave][lane], s_fs[wave][lane]);
+        __syncthreads();  // the next round reuses the LDS rows
+    }
+}
+
+// Appends the bodies of the lanes with `take` to the system's fallback list (one atomic
+// per wave).
+__device__ __forceinline__ void to_fallback(bool take, int b, SysState* S, int* fb) {
+    const unsigned long long m = __ballot(take);
+    if (!m) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)m) - 1;
+    int j0 = 0;
+    if (lane == leader) j0 = atomicAdd(&S->nfb, __popcll(m));
+    j0 = __shfl(j0, leader);
+    if (take) fb[j0 + __popcll(m & ((1ull << lane) - 1))] = b;
+}
+
+// Each entry of the frame list carries the body's position, id, bucket key and the level of
+// the bucket's cell, so that a frame build reads its entries in one batch of loads.
+__global__ __launch_bounds__(256) void k_scatter(const SysDesc* __restrict__ sd, SysState* st,
+                                                 const BlkWork* __restrict__ work, const float4* __restrict__ node4,
+                                                 const int* __restrict__ kbase, const int* __restrict__ bkey,
+                                                 const int* __restrict__ brank, const int* __restrict__ ccount,
+                                                 const int* __restrict__ sorder, float4* lpos, int4* lmeta, int* fblist,
+                                                 int stride, int exclude) {
+    const BlkWork w = work[blockIdx.x];
+    const SysDesc d = sd[w.sys];
+    SysState* S = st + w.sys;
+    if (S->status || d.algo != 0) return;
+    const int n = d.n;
+    const int i = w.begin + (int)threadIdx.x;  // the classify's enumeration
+    const int b = phase_item(w, S, sorder + d.body_off, n, stride, exclude);
+    if (b < 0) return;
+    const int key = bkey[d.body_off + i];
+    const int base = kbase[8 * (size_t)d.cell_off + key];
+    if (base >= 0) {
+        const int pos = base + brank[d.body_off + i];
+        if (pos < n) {
+            lpos[d.body_off + pos] = node4[d.node_off + b];
+            lmeta[d.body_off + pos] = make_int4(b, key, ccount[d.cell_off + (key >> 3)], 0);
+        }
+    }
+    to_fallback(base < 0, b, S, fblist + d.body_off);
+}
+
+struct FrameLds {
+    int c[8 * kFrameCells];  // slots: -1 empty, [0, kFrame) an entry of the frame, >= kCellTag a cell
+    int par[kFrameCells];    // (parent cell << 3) | slot; a bucket's root: -1 - key (its slot of T)
+    int cont[kFrameCells];   // bit k: slot k claimed by more than one body this level
+    int body[kFrame];        // entry -> body
+    int key[kFrame];         // entry -> bucket key; for a leader, later its bucket's root cell
+    int wmax[kFrameThreads / 64];
+    int nc, fail, base;
+};
+
+// One workgroup per frame (grid-stride over the system's frames): entries e = 4 * thread + t
+// in frame order, every bucket's subtree built at once, level by level.
+__global__ __launch_bounds__(kFrameThreads) void k_frame_build(
+    const SysDesc* __restrict__ sd, SysState* st, const BlkWork* __restrict__ work, const float4* __restrict__ node4,
+    int* child, int* cparent, int* cpend, int* slotoff, float4* cdata, const float4* __restrict__ lpos,
+    const int4* __restrict__ lmeta, const int2* __restrict__ bframe, int* fblist, int last) {
+    __shared__ FrameLds L;
+    const BlkWork w = work[blockIdx.x];
+    const SysDesc d = sd[w.sys];
+    SysState* S = st + w.sys;
+    if (S->status || d.algo != 0) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int n = d.n;
+    const int nf = min(S->nframes, n);  // k_frames keeps it within n (else the status is set)
+    int* C = child + 8 * (size_t)d.cell_off;
+    float4* CD = cdata + 8 * (size_t)d.cell_off;
+    float rlx, rly, rlz, rhx, rhy, rhz;
+    root_box(d.rules, S->box, rlx, rly, rlz, rhx, rhy, rhz);
+    int depth = 1;
+    for (int f = w.begin; f < nf; f += w.end) {  // uniform over the block
+        const int2 fr = bframe[d.body_off + f];
+        const int size = min(max(fr.y, 0), kFrame);
+        int eb[4], ek[4], node[4], lvl[4], slot[4], dgs[4], lv0[4];
+        float4 p[4];
+        float bx[4][6];
+        bool act[4], valid[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int e = 4 * tid + t;
+            valid[t] = e < size && fr.x + e < n;
+            int4 mt = make_int4(-1, -1, 0, 0);
+            p[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (valid[t]) {
+                mt = lmeta[d.body_off + fr.x + e];
+                p[t] = lpos[d.body_off + fr.x + e];
+            }
+            eb[t] = valid[t] ? min(max(mt.x, 0), n - 1) : -1;
+            ek[t] = valid[t] ? min(max(mt.y, 0), 8 * d.cap - 1) : -1;
+            lv0[t] = min(max(mt.z, 0), kMaxLevel);
+            L.key[4 * tid + t] = ek[t];
+            if (valid[t]) L.body[4 * tid + t] = eb[t];
+        }
+        if (tid == 0) {
+            L.nc = 0;
+            L.fail = 0;
+        }
+        __syncthreads();
+        // bucket leaders (the first entry of each key) and each entry's leader: an inclusive
+        // max-scan of (leader ? e : -1) over the frame (in the thread, the wave, the block)
+        bool lead[4], multi[4];
+        int li[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int e = 4 * tid + t;
+            const int prev = e > 0 ? L.key[e - 1] : -1;
+            const int next = e + 1 < size ? L.key[e + 1] : -1;
+            lead[t] = valid[t] && (e == 0 || prev != ek[t]);
+            multi[t] = valid[t] && (next == ek[t] || !lead[t]);
+            li[t] = lead[t] ? e : -1;
+            if (t) li[t] = max(li[t], li[t - 1]);
+        }
+        int scan = li[3];
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(scan, o);
+            if (lane >= o) scan = max(scan, y);
+        }
+        if (lane == 63) L.wmax[wave] = scan;
+        int before = __shfl_up(scan, 1);
+        if (lane == 0) before = -1;
+        __syncthreads();
+        for (int q = 0; q < wave; ++q) before = max(before, L.wmax[q]);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) li[t] = max(li[t], before);
+        // each entry's box down to its bucket's slot of T (the level of the slot's cell came
+        // from k_classify); the bucket leaders open the bucket roots
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            act[t] = false;
+            lvl[t] = lv0[t] + 1;
+            if (!valid[t]) continue;
+            float lx = rlx, ly = rly, lz = rlz, hx = rhx, hy = rhy, hz = rhz;
+            int dg = 0;
+            for (int l = 0; l <= lv0[t]; ++l) dg = level_digit(d.rules, l, p[t].x, p[t].y, p[t].z, lx, ly, lz, hx, hy, hz);
+            bx[t][0] = lx; bx[t][1] = ly; bx[t][2] = lz; bx[t][3] = hx; bx[t][4] = hy; bx[t][5] = hz;
+            if (dg != (ek[t] & 7)) L.fail = 2;  // not the slot the descent ended at: cannot happen
+            if (lead[t] && multi[t]) {
+                const int c = min(atomicAdd(&L.nc, 1), kFrameCells - 1);  // at most kFrame / 2 roots
+                for (int k = 0; k < 8; ++k) L.c[8 * c + k] = kEmpty;
+                L.par[c] = -1 - ek[t];
+                L.cont[c] = 0;
+                L.key[4 * tid + t] = c;  // read only by this bucket's entries from here on
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            if (multi[t]) {
+                node[t] = min(max(L.key[min(max(li[t], 0), kFrame - 1)], 0), kFrameCells - 1);
+                act[t] = true;
+            }
+        __syncthreads();
+        // level by level, every bucket's subtree at once
+        while (__syncthreads_or(act[0] || act[1] || act[2] || act[3]) && !L.fail) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                if (!act[t]) continue;
+                if (lvl[t] > kMaxLevel) {
+                    L.fail = 1;  // the depth limit: the insertion reports it
+                    act[t] = false;
+                    continue;
+                }
+                float lx = bx[t][0], ly = bx[t][1], lz = bx[t][2], hx = bx[t][3], hy = bx[t][4], hz = bx[t][5];
+                const int dg = level_digit(d.rules, lvl[t], p[t].x, p[t].y, p[t].z, lx, ly, lz, hx, hy, hz);
+                bx[t][0] = lx; bx[t][1] = ly; bx[t][2] = lz; bx[t][3] = hx; bx[t][4] = hy; bx[t][5] = hz;
+                dgs[t] = dg;
+                slot[t] = 8 * node[t] + dg;
+                if (atomicCAS(&L.c[slot[t]], kEmpty, 4 * tid + t) != kEmpty) atomicOr(&L.cont[node[t]], 1 << dg);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                if (!act[t]) continue;
+                if (!((L.cont[node[t]] >> dgs[t]) & 1)) {  // alone in its slot: placed
+                    act[t] = false;
+                    depth = max(depth, lvl[t]);
+                } else if (L.c[slot[t]] == 4 * tid + t) {  // the holder of a contested slot opens a cell
+                    const int c = atomicAdd(&L.nc, 1);
+                    if (c >= kFrameCells) {
+                        L.fail = 1;
+                    } else {
+                        for (int k = 0; k < 8; ++k) L.c[8 * c + k] = kEmpty;
+                        L.par[c] = (node[t] << 3) | dgs[t];
+                        L.cont[c] = 0;
+                        L.c[slot[t]] = kCellTag + c;
+                    }
+                }
+            }
+            __syncthreads();
+            if (L.fail) break;  // uniform: read between two barriers
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                if (act[t]) {  // into the new cell, one level down
+                    node[t] = min(max(L.c[slot[t]] - kCellTag, 0), kFrameCells - 1);
+                    ++lvl[t];
+                }
+        }
+        __syncthreads();
+        const int fail = L.fail;
+        if (fail == 2 && tid == 0) atomicOr(&S->status, kStStall);
+        if (fail) {
+            // nothing of this frame was written: its bodies go to the insertion, except the
+            // ones already in their slots of T
+#pragma unroll
+            for (int t = 0; t < 4; ++t) to_fallback(valid[t] && C[ek[t]] != eb[t], eb[t], S, fblist + d.body_off);
+            __syncthreads();
+            continue;
+        }
+        const int nc = L.nc;
+        if (tid == 0) L.base = nc > 0 ? atomicAdd(&S->ncells, nc) : 0;
+        __syncthreads();
+        const int base = L.base;
+        if (base + nc > d.cap) {
+            if (tid == 0) atomicOr(&S->status, kStTreeOverflow);
+            __syncthreads();
+            continue;
+        }
+        // the cells: slots, parent links, counts of child cells, cleared per-slot counts
+        for (int c = tid; c < nc; c += kFrameThreads) {
+            const int g = base + c;
+            int v[8], np = 0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int sv = L.c[8 * c + k];
+                v[k] = sv == kEmpty ? kEmpty : (sv >= kCellTag ? n + base + (sv - kCellTag) : L.body[min(sv, kFrame - 1)]);
+                np += sv >= kCellTag;
+            }
+            reinterpret_cast<int4*>(C + 8 * (size_t)g)[0] = make_int4(v[0], v[1], v[2], v[3]);
+            reinterpret_cast<int4*>(C + 8 * (size_t)g)[1] = make_int4(v[4], v[5], v[6], v[7]);
+            cpend[d.cell_off + g] = np;
+            const int pv = L.par[c];
+            cparent[d.cell_off + g] = pv < 0 ? (-1 - pv) : (((base + (pv >> 3)) << 3) | (pv & 7));
+            if (!last) {  // a later phase counts its buckets here
+                int4* so = reinterpret_cast<int4*>(slotoff + 8 * ((size_t)d.cell_off + g));
+                so[0] = make_int4(0, 0, 0, 0);
+                so[1] = make_int4(0, 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            if (!valid[t]) continue;
+            if (multi[t]) {  // placed in a new cell: the body's entry of that cell's child block
+                CD[8 * (size_t)(base + node[t]) + dgs[t]] = p[t];
+                if (lead[t]) {  // the subtree takes the bucket's slot of T
+                    C[ek[t]] = n + base + L.key[4 * tid + t];
+                    atomicAdd(cpend + d.cell_off + (ek[t] >> 3), 1);
+                }
+            } else {  // alone in its bucket: straight into the slot
+                C[ek[t]] = eb[t];
+                CD[ek[t]] = p[t];
+                depth = max(depth, lv0[t]);
+            }
+        }
+        __syncthreads();  // the next frame reuses the LDS
+    }
+    depth = wave_max_i(depth);
+    if (lane == 0 && depth > ld_c(&S->max_depth)) atomicMax(&S->max_depth, depth);
+}
+
+void launch_classify(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
+                     const float4* node4, const int* child, int* slotoff, int* ccount, int* bkey, int* brank,
+                     const int* sorder, int stride, int exclude) {
+    if (nblk > 0)
+        hipLaunchKernelGGL(k_classify, dim3(nblk), dim3(256), 0, s, sd, st, work, node4, child, slotoff, ccount, bkey,
+                           brank, sorder, stride, exclude);
+}
+void launch_frames(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
+                   const float4* node4, const int* child, int* slotoff, const int* ccount, int* kbase, float4* lpos,
+                   int4* lmeta, int2* bframe) {
+    if (nblk > 0)
+        hipLaunchKernelGGL(k_frames, dim3(nblk), dim3(256), 0, s, sd, st, work, node4, child, slotoff, ccount, kbase,
+                           lpos, lmeta, bframe);
+}
+void launch_scatter(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
+                    const float4* node4, const int* kbase, const int* bkey, const int* brank, const int* ccount,
+                    const int* sorder, float4* lpos, int4* lmeta, int* fblist, int stride, int exclude) {
+    if (nblk > 0)
+        hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(256), 0, s, sd, st, work, node4, kbase, bkey, brank, ccount,
+                           sorder, lpos, lmeta, fblist, stride, exclude);
+}
+void launch_frame_build(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
+                        const float4* node4, int* child, int* cparent, int* cpend, int* slotoff, float4* cdata,
+                        const float4* lpos, const int4* lmeta, const int2* bframe, int* fblist, bool last) {
+    if (nblk > 0)
+        hipLaunchKernelGGL(k_frame_build, dim3(nblk), dim3(kFrameThreads), 0, s, sd, st, work, node4, child,
+                           cparent, cpend, slotoff, cdata, lpos, lmeta, bframe, fblist, last ? 1 : 0);
+}
+
+}  // namespace nb

@@ -23,12 +23,29 @@ void launch_bounds(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, con
                    const float4* node4, float* partial);
 void launch_tree_clear(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
                        int* child, int* cpend);
-void launch_tree_init(hipStream_t s, int nsys, const int* slots, SysState* st);  // slots NULL: 0..nsys-1
+// slots NULL: 0..nsys-1; sd and slotoff NULL: the roots' per-slot counts are left alone
+void launch_tree_init(hipStream_t s, int nsys, const int* slots, const SysDesc* sd, SysState* st, int* slotoff);
 void launch_build_top(hipStream_t s, int nsys, const int* slots, const SysDesc* sd, SysState* st,
-                      const float4* node4, int* child, int* cparent, int* cpend, float4* cdata, int stride);
+                      const float4* node4, int* child, int* cparent, int* cpend, float4* cdata, int* slotoff,
+                      int stride);
+// list: the queued phase takes its bodies from this per-system list (SysState::nfb entries)
 void launch_build(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
-                  const float4* node4, int* child, int* cparent, int* cpend, float4* cdata, int stride,
-                  int exclude);
+                  const float4* node4, int* child, int* cparent, int* cpend, float4* cdata, int* slotoff, int stride,
+                  int exclude, const int* list);
+// the bucketed insertion phases (nb_build.hip)
+void launch_classify(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
+                     const float4* node4, const int* child, int* slotoff, int* ccount, int* bkey, int* brank,
+                     const int* sorder, int stride, int exclude);
+void launch_frames(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
+                   const float4* node4, const int* child, int* slotoff, const int* ccount, int* kbase, float4* lpos,
+                   int4* lmeta, int2* bframe);
+void launch_scatter(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
+                    const float4* node4, const int* kbase, const int* bkey, const int* brank, const int* ccount,
+                    const int* sorder, float4* lpos, int4* lmeta, int* fblist, int stride, int exclude);
+// last: no later bucketed phase (the new cells' per-slot counts are not cleared)
+void launch_frame_build(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
+                        const float4* node4, int* child, int* cparent, int* cpend, int* slotoff, float4* cdata,
+                        const float4* lpos, const int4* lmeta, const int2* bframe, int* fblist, bool last);
 void launch_summarize(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
                       float4* node4, const int* child, const int* cparent, int* cpend, int* ccount, int* slotoff,
                       float4* cdata);
@@ -97,8 +114,8 @@ struct Capacity {
 
 // Bytes of device memory per node, body and cell in the arena layout.
 constexpr size_t kBytesPerNode = 16;                     // node4
-constexpr size_t kBytesPerBody = 16 + 16 + 4 + 4 + 4;    // vel4, acc4, sorted, forder, sorder
-constexpr size_t kBytesPerCell = 32 + 4 * 3 + 32 + 128;  // child, parent/pend/count, slotoff, cdata
+constexpr size_t kBytesPerBody = 16 + 16 + 4 + 4 + 4 + 32 + 8 + 4;  // vel4, acc4, sorted, forder, sorder, lpos+lmeta, bframe, fblist
+constexpr size_t kBytesPerCell = 32 + 4 * 3 + 32 + 128 + 32;      // child, parent/pend/count, slotoff, cdata, kbase
 
 struct Engine {
 #ifndef NB_BUILD_PHASES
@@ -130,6 +147,12 @@ struct Engine {
     float4 *node4 = nullptr, *vel4 = nullptr, *acc4 = nullptr, *cdata = nullptr;
     int *sorted = nullptr, *forder = nullptr, *sorder = nullptr, *child = nullptr, *cparent = nullptr, *cpend = nullptr, *ccount = nullptr,
         *slotoff = nullptr;
+    // build scratch of the bucketed phases: per slot a bucket's place in the frame list; the
+    // frame list (position; body, key, level), frames, fallback list
+    int *kbase = nullptr, *fblist = nullptr;
+    float4* lpos = nullptr;
+    int4* lmeta = nullptr;
+    int2* bframe = nullptr;
     SysDesc* d_desc = nullptr;
     SysState* d_state = nullptr;
     float* d_partial = nullptr;
@@ -141,8 +164,9 @@ struct Engine {
     // block tables (device) with their capacities and current lengths
     // T_RANGE (target-range force launches) and T_FORDER (T_GROUPS in dispatch
     // order) are filled on the device
+    // T_KEYS, T_FRAMES: (system, block index in the system, the system's block count)
     enum Table { T_BOUNDS, T_BODY, T_CELLS, T_GROUPS, T_BRUTE, T_UPDATE, T_PHASE0, T_PHASE1, T_PHASE2, T_PHASE3,
-                 T_RANGE, T_FORDER, T_COUNT };
+                 T_RANGE, T_FORDER, T_KEYS, T_FRAMES, T_COUNT };
     BlkWork* tab[T_COUNT] = {};
     int tab_cap[T_COUNT] = {};
     int tab_len[T_COUNT] = {};

@@ -18,6 +18,10 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 #endif
 static constexpr int kPhaseStride[Engine::kBuildPhases] = {NB_PHASE_STRIDES};
 static constexpr int kBoundsPerBlock = 2048;
+// Bucketed phases: blocks per system of the frame packing (over the slots of the tree so
+// far) and of the frame builds (one workgroup per frame in flight; frames hold up to 512 bodies)
+static constexpr int kKeysBodiesPerBlock = 4096;
+static constexpr int kFramesBodiesPerBlock = 512;
 
 #define NB_HIP(call)                                                                   \
     do {                                                                               \
@@ -136,6 +140,8 @@ nb_status Engine::reserve(const Capacity& c, int law_, bool kahan_, void* user_s
     tab_cap[T_UPDATE] = blocks_bound(c.bodies, 256, S);
     tab_cap[T_RANGE] = blocks_bound(c.bodies, 16, S);  // the smallest share per block of either force kernel
     tab_cap[T_FORDER] = tab_cap[T_GROUPS];
+    tab_cap[T_KEYS] = blocks_bound(c.bodies, kKeysBodiesPerBlock, S);
+    tab_cap[T_FRAMES] = blocks_bound(c.bodies, kFramesBodiesPerBlock, S);
     for (int ph = 0; ph < kBuildPhases; ++ph)
         tab_cap[T_PHASE0 + ph] =
             blocks_bound((c.bodies + kPhaseStride[ph] - 1) / kPhaseStride[ph], build_block(kPhaseStride[ph]), S);
@@ -156,6 +162,11 @@ nb_status Engine::reserve(const Capacity& c, int law_, bool kahan_, void* user_s
         {(void**)&ccount, sizeof(int) * nc},
         {(void**)&slotoff, sizeof(int) * 8 * nc},
         {(void**)&cdata, sizeof(float4) * 8 * nc},
+        {(void**)&kbase, sizeof(int) * 8 * nc},
+        {(void**)&lpos, sizeof(float4) * nb},
+        {(void**)&lmeta, sizeof(int4) * nb},
+        {(void**)&bframe, sizeof(int2) * nb},
+        {(void**)&fblist, sizeof(int) * nb},
         {(void**)&d_desc, sizeof(SysDesc) * S},
         {(void**)&d_state, sizeof(SysState) * S},
         {(void**)&d_partial, sizeof(float) * 7 * (size_t)tab_cap[T_BOUNDS]},
@@ -185,7 +196,7 @@ nb_status Engine::reserve(const Capacity& c, int law_, bool kahan_, void* user_s
     NB_HIP(hipMemsetAsync(slotoff, 0, sizeof(int) * 8 * nc, stream));
     NB_HIP(hipMemsetAsync(d_state, 0, sizeof(SysState) * S, stream));
     NB_HIP(hipMemsetAsync(gkey, 0, sizeof(int) * (size_t)std::max(tab_cap[T_GROUPS], 1), stream));
-    launch_tree_init(stream, S, nullptr, d_state);
+    launch_tree_init(stream, S, nullptr, nullptr, d_state, nullptr);  // slotoff is cleared above
     NB_HIP(hipGetLastError());
     NB_HIP(hipStreamSynchronize(stream));
     return NB_OK;
@@ -266,6 +277,11 @@ nb_status Engine::activate(const std::vector<Slot>& slots) {
                 }
                 add_ranges(t[T_PHASE0 + ph], s, (int)items, blk);
             }
+            // grid-stride tables of the bucketed phases: block i of the system's k
+            const int nk = std::max(1, (d.n + kKeysBodiesPerBlock - 1) / kKeysBodiesPerBlock);
+            for (int i = 0; i < nk; ++i) t[T_KEYS].push_back(BlkWork{s, i, nk, 0});
+            const int nfr = std::max(1, (d.n + kFramesBodiesPerBlock - 1) / kFramesBodiesPerBlock);
+            for (int i = 0; i < nfr; ++i) t[T_FRAMES].push_back(BlkWork{s, i, nfr, 0});
         }
         if (!bh && d.n > 0) {
             any_bf = true;
@@ -356,14 +372,34 @@ void Engine::bounds() {
 void Engine::tree_reset() {
     TraceRange tr("nb:tree_reset");
     launch_tree_clear(stream, tab_len[T_CELLS], d_desc, d_state, tab[T_CELLS], child, cpend);
-    launch_tree_init(stream, nactive, d_slots, d_state);
+    launch_tree_init(stream, nactive, d_slots, d_desc, d_state, slotoff);
 }
+// The first phase (every 4096th body) in LDS (k_build_top), or by insertion when it does not
+// fit there; each later phase bucketed (nb_build.hip) against the tree of the phases before
+// it, with the queued insertion taking the bodies that the bucketed kernels leave (buckets
+// over 256 bodies, subtrees over the LDS pool or at the depth limit).
 void Engine::build() {
     TraceRange tr("nb:build");
-    launch_build_top(stream, nactive, d_slots, d_desc, d_state, node4, child, cparent, cpend, cdata, kPhaseStride[0]);
-    for (int ph = 0; ph < kBuildPhases; ++ph)
-        launch_build(stream, tab_len[T_PHASE0 + ph], d_desc, d_state, tab[T_PHASE0 + ph], node4, child, cparent,
-                     cpend, cdata, kPhaseStride[ph], ph == 0 ? 0 : kPhaseStride[ph - 1]);
+    launch_build_top(stream, nactive, d_slots, d_desc, d_state, node4, child, cparent, cpend, cdata, slotoff,
+                     kPhaseStride[0]);
+    launch_build(stream, tab_len[T_PHASE0], d_desc, d_state, tab[T_PHASE0], node4, child, cparent, cpend, cdata,
+                 slotoff, kPhaseStride[0], 0, nullptr);
+    for (int ph = 1; ph < kBuildPhases; ++ph) {
+        // the phase's bodies: a strided table, or every body (skipping the earlier phases')
+        const bool strided = kPhaseStride[ph] > 1;
+        const Table bt = strided ? (Table)(T_PHASE0 + ph) : T_BODY;
+        const int stride = kPhaseStride[ph], exclude = kPhaseStride[ph - 1];
+        launch_classify(stream, tab_len[bt], d_desc, d_state, tab[bt], node4, child, slotoff, ccount, sorted, forder,
+                        sorder, stride, exclude);
+        launch_frames(stream, tab_len[T_KEYS], d_desc, d_state, tab[T_KEYS], node4, child, slotoff, ccount, kbase,
+                      lpos, lmeta, bframe);
+        launch_scatter(stream, tab_len[bt], d_desc, d_state, tab[bt], node4, kbase, sorted, forder, ccount, sorder,
+                       lpos, lmeta, fblist, stride, exclude);
+        launch_frame_build(stream, tab_len[T_FRAMES], d_desc, d_state, tab[T_FRAMES], node4, child, cparent, cpend,
+                           slotoff, cdata, lpos, lmeta, bframe, fblist, ph == kBuildPhases - 1);
+        launch_build(stream, tab_len[T_PHASE0 + kBuildPhases - 1], d_desc, d_state,
+                     tab[T_PHASE0 + kBuildPhases - 1], node4, child, cparent, cpend, cdata, slotoff, 1, 0, fblist);
+    }
 }
 void Engine::summarize() {
     TraceRange tr("nb:summarize");

@@ -20,6 +20,10 @@
 //   cparent, cpend, ccount           int[sum cap_s]   parent, unfinished child cells, bodies below
 //   slotoff                          int[8 * sum cap_s] bodies in slots 0..k of the same cell
 //   cdata                            float4[8 * sum cap_s] children (x,y,z,m) in slot order
+//   kbase                            int[8 * sum cap_s] build scratch: a bucket's place in the frame list
+//   lpos float4[sum n_s], lmeta int4[sum n_s], bframe int2[sum n_s], fblist int[sum n_s]
+//                                    build scratch: frame list (position; body, key, level),
+//                                    frames (list begin, size), fallback list
 #pragma once
 #include <stdint.h>
 
@@ -90,6 +94,12 @@ struct SysState {
     int order_ok;   // 1: sorder holds a permutation of the bodies (the last full force pass's order)
     int top;        // 1: this step's first insertion phase was built in LDS (k_build_top)
     int pad1;
+    // a bucketed build phase: frames and frame-list entries claimed (the two halves of one
+    // 8-byte-aligned word, claimed together), and bodies sent to the fallback insertion
+    int nframes;
+    int listpos;
+    int nfb;
+    int pad2;
 };
 
 // One workgroup's share of one system: items [begin, end).  `trim` is only set

@@ -153,10 +153,16 @@ __global__ __launch_bounds__(256) void k_tree_clear(const SysDesc* __restrict__ 
     cpend[d.cell_off + ci] = 0;
 }
 
-__global__ void k_tree_init(int nsys, const int* __restrict__ slots, SysState* st) {
+__global__ void k_tree_init(int nsys, const int* __restrict__ slots, const SysDesc* __restrict__ sd, SysState* st,
+                            int* slotoff) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nsys) return;
     const int s = slots ? slots[k] : k;
+    if (sd && slotoff && sd[s].algo == 0 && sd[s].cap > 0) {  // the root's per-slot counts start at zero
+        int4* so = reinterpret_cast<int4*>(slotoff + 8 * (size_t)sd[s].cell_off);
+        so[0] = make_int4(0, 0, 0, 0);
+        so[1] = make_int4(0, 0, 0, 0);
+    }
     st[s].ncells = 1;
     st[s].max_depth = 1;  // build_tree.cl:28
     st[s].bq = 0;
@@ -201,21 +207,25 @@ template <int BLK>  // workgroup size: build_block(stride)
 __global__ __launch_bounds__(BLK) void k_build(const SysDesc* __restrict__ sd, SysState* st,
                                                const BlkWork* __restrict__ work,
                                                const float4* __restrict__ node4, int* child, int* cparent,
-                                               int* cpend, float4* cdata, int stride, int exclude) {
+                                               int* cpend, float4* cdata, int* slotoff, int stride, int exclude,
+                                               const int* __restrict__ list) {
     const BlkWork w = work[blockIdx.x];
     const SysDesc d = sd[w.sys];
     SysState* S = st + w.sys;
     if (S->status) return;
-    if (exclude == 0 && S->top) return;  // the first phase was built in LDS (k_build_top)
+    if (exclude == 0 && !list && S->top) return;  // the first phase was built in LDS (k_build_top)
     const int n = d.n;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     // The last phase (stride 1) takes its bodies from a per-system queue: a lane whose
     // insertion is complete claims the next body at the round's allocation, so that no
     // lane idles while others of its workgroup retry under contention.  Queue entry j
-    // is the j-th body that the earlier phases left out (not a multiple of `exclude`).
+    // is the j-th body that the earlier phases left out (not a multiple of `exclude`),
+    // or, with a list (the bucketed phases' fallback), the list's j-th body.
     const bool queued = stride == 1;
-    const int nq = exclude > 1 ? n - (n + exclude - 1) / exclude : (exclude == 1 ? 0 : n);
+    const int nq = list ? min(max(ld_c(&S->nfb), 0), n)
+                        : (exclude > 1 ? n - (n + exclude - 1) / exclude : (exclude == 1 ? 0 : n));
+    const int* blist = list ? list + d.body_off : nullptr;
     int* C = child + 8 * (size_t)d.cell_off;
     int* par = cparent + d.cell_off;
     int* pend = cpend + d.cell_off;
@@ -252,7 +262,9 @@ __global__ __launch_bounds__(BLK) void k_build(const SysDesc* __restrict__ sd, S
         for (int q = 0; q < wave; ++q) off += s_widle[q];
         return off;
     };
-    auto body_of = [&](int j) { return exclude > 1 ? j + j / (exclude - 1) + 1 : j; };
+    auto body_of = [&](int j) {
+        return blist ? min(max(blist[j], 0), n - 1) : (exclude > 1 ? j + j / (exclude - 1) + 1 : j);
+    };
     bool drained = false;  // thread 0: the queue has no entries left
     if (queued) {
         if (lane == 0) s_widle[wave] = 64;
@@ -374,6 +386,10 @@ __global__ __launch_bounds__(BLK) void k_build(const SysDesc* __restrict__ sd, S
                     const int dq2 = level_digit(d.rules, l2, q4.x, q4.y, q4.z, qx0, qy0, qz0, qx1, qy1, qz1);
                     par[c] = (parent << 3) | pslot;  // parent cell and the slot holding c
                     st_c(pend + c, (i < k - 1) ? 1 : 0);
+                    // per-slot counts of a new cell start at zero (a later bucketed phase counts there)
+                    int4* so = reinterpret_cast<int4*>(slotoff + 8 * ((size_t)d.cell_off + c));
+                    so[0] = make_int4(0, 0, 0, 0);
+                    so[1] = make_int4(0, 0, 0, 0);
                     if (i < k - 1) {
                         st_c(C + 8 * c + db2, n + c + 1);
                     } else {
@@ -432,7 +448,7 @@ constexpr int kTopBodies = 512;  // 8 KiB of LDS: the phase's bodies
 __global__ __launch_bounds__(kTopBodies) void k_build_top(int nsys, const int* __restrict__ slots,
                                                           const SysDesc* __restrict__ sd, SysState* st,
                                                           const float4* __restrict__ node4, int* child, int* cparent,
-                                                          int* cpend, float4* cdata, int stride) {
+                                                          int* cpend, float4* cdata, int* slotoff, int stride) {
     __shared__ int sC[8 * kTopCells];
     __shared__ int sPar[kTopCells];
     __shared__ int sCont[kTopCells];  // bit k: slot k was claimed by more than one body this level
@@ -513,6 +529,9 @@ __global__ __launch_bounds__(kTopBodies) void k_build_top(int nsys, const int* _
         cpend[d.cell_off + c] = (a.x >= n) + (a.y >= n) + (a.z >= n) + (a.w >= n) + (e.x >= n) + (e.y >= n) +
                                 (e.z >= n) + (e.w >= n);
         if (c > 0) cparent[d.cell_off + c] = sPar[c];
+        int4* so = reinterpret_cast<int4*>(slotoff + 8 * ((size_t)d.cell_off + c));  // counts of a later phase
+        so[0] = make_int4(0, 0, 0, 0);
+        so[1] = make_int4(0, 0, 0, 0);
     }
     if (threadIdx.x == 0) {
         S->ncells = nc;
@@ -1337,27 +1356,29 @@ void launch_tree_clear(hipStream_t s, int nblk, const SysDesc* sd, const SysStat
                        int* child, int* cpend) {
     NB_LAUNCH(k_tree_clear, nblk, 256, s, sd, st, work, child, cpend);
 }
-void launch_tree_init(hipStream_t s, int nsys, const int* slots, SysState* st) {
+void launch_tree_init(hipStream_t s, int nsys, const int* slots, const SysDesc* sd, SysState* st, int* slotoff) {
     if (nsys <= 0) return;
-    NB_LAUNCH(k_tree_init, (nsys + 255) / 256, 256, s, nsys, slots, st);
+    NB_LAUNCH(k_tree_init, (nsys + 255) / 256, 256, s, nsys, slots, sd, st, slotoff);
 }
 void launch_build_top(hipStream_t s, int nsys, const int* slots, const SysDesc* sd, SysState* st,
-                      const float4* node4, int* child, int* cparent, int* cpend, float4* cdata, int stride) {
+                      const float4* node4, int* child, int* cparent, int* cpend, float4* cdata, int* slotoff,
+                      int stride) {
     if (nsys <= 0) return;
-    NB_LAUNCH(k_build_top, nsys, kTopBodies, s, nsys, slots, sd, st, node4, child, cparent, cpend, cdata, stride);
+    NB_LAUNCH(k_build_top, nsys, kTopBodies, s, nsys, slots, sd, st, node4, child, cparent, cpend, cdata, slotoff,
+              stride);
 }
 void launch_build(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
-                  const float4* node4, int* child, int* cparent, int* cpend, float4* cdata, int stride,
-                  int exclude) {
+                  const float4* node4, int* child, int* cparent, int* cpend, float4* cdata, int* slotoff, int stride,
+                  int exclude, const int* list) {
     if (build_block(stride) == kBuildBlockSparse)
         NB_LAUNCH(k_build<kBuildBlockSparse>, nblk, kBuildBlockSparse, s, sd, st, work, node4, child, cparent, cpend,
-                  cdata, stride, exclude);
+                  cdata, slotoff, stride, exclude, list);
     else if (build_block(stride) == kBuildBlockStrided)
         NB_LAUNCH(k_build<kBuildBlockStrided>, nblk, kBuildBlockStrided, s, sd, st, work, node4, child, cparent, cpend,
-                  cdata, stride, exclude);
+                  cdata, slotoff, stride, exclude, list);
     else
         NB_LAUNCH(k_build<kBuildBlock>, nblk, kBuildBlock, s, sd, st, work, node4, child, cparent, cpend, cdata,
-                  stride, exclude);
+                  slotoff, stride, exclude, list);
 }
 void launch_summarize(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
                       float4* node4, const int* child, const int* cparent, int* cpend, int* ccount, int* slotoff,
