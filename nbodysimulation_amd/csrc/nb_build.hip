@@ -24,8 +24,10 @@
 // reach the depth limit, go to k_build's queued insertion (list mode) afterwards.
 //
 // Scratch during the build (each array is written afresh by a later stage):
-//   sorted / forder   per body: bucket key and rank (the sort writes both permutations)
-// and of its own: the frame list (per entry the body's position; its id, bucket key and the
+//   sorted / forder   per item of the phase's enumeration: bucket key and rank (the sort
+//                     writes both permutations)
+// and of its own: per item the body's position (ppos), the frame list (per entry the body's
+// position; its id, bucket key and the
 // level of the bucket's cell), the frames (list begin, size), the fallback list, and per slot
 // of T the bucket's place in the frame list (kbase)
 //   slotoff           per slot of T: the bucket's body count, cleared by k_frames (summarize
@@ -78,7 +80,7 @@ __device__ __forceinline__ int phase_item(const BlkWork& w, const SysState* S, c
 __global__ __launch_bounds__(256) void k_classify(const SysDesc* __restrict__ sd, SysState* st,
                                                   const BlkWork* __restrict__ work, const float4* __restrict__ node4,
                                                   const int* __restrict__ child, int* slotoff, int* ccount, int* bkey,
-                                                  int* brank, const int* __restrict__ sorder, int stride,
+                                                  int* brank, float4* ppos, const int* __restrict__ sorder, int stride,
                                                   int exclude) {
     const BlkWork w = work[blockIdx.x];
     const SysDesc d = sd[w.sys];
@@ -143,6 +145,7 @@ __global__ __launch_bounds__(256) void k_classify(const SysDesc* __restrict__ sd
     }
     brank[d.body_off + i] = rank;
     bkey[d.body_off + i] = key;
+    ppos[d.body_off + i] = p;  // for k_scatter, in the same enumeration (a coalesced read there)
     ccount[d.cell_off + node] = lvl;  // the same value from every body that ends here
 }
 
@@ -268,7 +271,7 @@ __device__ __forceinline__ void to_fallback(bool take, int b, SysState* S, int* 
 // Each entry of the frame list carries the body's position, id, bucket key and the level of
 // the bucket's cell, so that a frame build reads its entries in one batch of loads.
 __global__ __launch_bounds__(256) void k_scatter(const SysDesc* __restrict__ sd, SysState* st,
-                                                 const BlkWork* __restrict__ work, const float4* __restrict__ node4,
+                                                 const BlkWork* __restrict__ work, const float4* __restrict__ ppos,
                                                  const int* __restrict__ kbase, const int* __restrict__ bkey,
                                                  const int* __restrict__ brank, const int* __restrict__ ccount,
                                                  const int* __restrict__ sorder, float4* lpos, int4* lmeta, int* fblist,
@@ -286,7 +289,7 @@ __global__ __launch_bounds__(256) void k_scatter(const SysDesc* __restrict__ sd,
     if (base >= 0) {
         const int pos = base + brank[d.body_off + i];
         if (pos < n) {
-            lpos[d.body_off + pos] = node4[d.node_off + b];
+            lpos[d.body_off + pos] = ppos[d.body_off + i];
             lmeta[d.body_off + pos] = make_int4(b, key, ccount[d.cell_off + (key >> 3)], 0);
         }
     }
@@ -514,10 +517,10 @@ __global__ __launch_bounds__(kFrameThreads) void k_frame_build(
 
 void launch_classify(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
                      const float4* node4, const int* child, int* slotoff, int* ccount, int* bkey, int* brank,
-                     const int* sorder, int stride, int exclude) {
+                     float4* ppos, const int* sorder, int stride, int exclude) {
     if (nblk > 0)
         hipLaunchKernelGGL(k_classify, dim3(nblk), dim3(256), 0, s, sd, st, work, node4, child, slotoff, ccount, bkey,
-                           brank, sorder, stride, exclude);
+                           brank, ppos, sorder, stride, exclude);
 }
 void launch_frames(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
                    const float4* node4, const int* child, int* slotoff, const int* ccount, int* kbase, float4* lpos,
@@ -527,10 +530,10 @@ void launch_frames(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, con
                            lpos, lmeta, bframe);
 }
 void launch_scatter(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
-                    const float4* node4, const int* kbase, const int* bkey, const int* brank, const int* ccount,
+                    const float4* ppos, const int* kbase, const int* bkey, const int* brank, const int* ccount,
                     const int* sorder, float4* lpos, int4* lmeta, int* fblist, int stride, int exclude) {
     if (nblk > 0)
-        hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(256), 0, s, sd, st, work, node4, kbase, bkey, brank, ccount,
+        hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(256), 0, s, sd, st, work, ppos, kbase, bkey, brank, ccount,
                            sorder, lpos, lmeta, fblist, stride, exclude);
 }
 void launch_frame_build(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,

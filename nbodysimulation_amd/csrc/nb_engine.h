@@ -35,12 +35,12 @@ void launch_build(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, cons
 // the bucketed insertion phases (nb_build.hip)
 void launch_classify(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
                      const float4* node4, const int* child, int* slotoff, int* ccount, int* bkey, int* brank,
-                     const int* sorder, int stride, int exclude);
+                     float4* ppos, const int* sorder, int stride, int exclude);
 void launch_frames(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
                    const float4* node4, const int* child, int* slotoff, const int* ccount, int* kbase, float4* lpos,
                    int4* lmeta, int2* bframe);
 void launch_scatter(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
-                    const float4* node4, const int* kbase, const int* bkey, const int* brank, const int* ccount,
+                    const float4* ppos, const int* kbase, const int* bkey, const int* brank, const int* ccount,
                     const int* sorder, float4* lpos, int4* lmeta, int* fblist, int stride, int exclude);
 // last: no later bucketed phase (the new cells' per-slot counts are not cleared)
 void launch_frame_build(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
@@ -114,7 +114,7 @@ struct Capacity {
 
 // Bytes of device memory per node, body and cell in the arena layout.
 constexpr size_t kBytesPerNode = 16;                     // node4
-constexpr size_t kBytesPerBody = 16 + 16 + 4 + 4 + 4 + 32 + 8 + 4;  // vel4, acc4, sorted, forder, sorder, lpos+lmeta, bframe, fblist
+constexpr size_t kBytesPerBody = 16 + 16 + 4 + 4 + 4 + 32 + 16 + 8 + 4;  // vel4, acc4, sorted, forder, sorder, lpos+lmeta, ppos, bframe, fblist
 constexpr size_t kBytesPerCell = 32 + 4 * 3 + 32 + 128 + 32;      // child, parent/pend/count, slotoff, cdata, kbase
 
 struct Engine {
@@ -150,7 +150,7 @@ struct Engine {
     // build scratch of the bucketed phases: per slot a bucket's place in the frame list; the
     // frame list (position; body, key, level), frames, fallback list
     int *kbase = nullptr, *fblist = nullptr;
-    float4* lpos = nullptr;
+    float4 *lpos = nullptr, *ppos = nullptr;
     int4* lmeta = nullptr;
     int2* bframe = nullptr;
     SysDesc* d_desc = nullptr;

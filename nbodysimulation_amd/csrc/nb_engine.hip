@@ -164,6 +164,7 @@ nb_status Engine::reserve(const Capacity& c, int law_, bool kahan_, void* user_s
         {(void**)&cdata, sizeof(float4) * 8 * nc},
         {(void**)&kbase, sizeof(int) * 8 * nc},
         {(void**)&lpos, sizeof(float4) * nb},
+        {(void**)&ppos, sizeof(float4) * nb},
         {(void**)&lmeta, sizeof(int4) * nb},
         {(void**)&bframe, sizeof(int2) * nb},
         {(void**)&fblist, sizeof(int) * nb},
@@ -390,10 +391,10 @@ void Engine::build() {
         const Table bt = strided ? (Table)(T_PHASE0 + ph) : T_BODY;
         const int stride = kPhaseStride[ph], exclude = kPhaseStride[ph - 1];
         launch_classify(stream, tab_len[bt], d_desc, d_state, tab[bt], node4, child, slotoff, ccount, sorted, forder,
-                        sorder, stride, exclude);
+                        ppos, sorder, stride, exclude);
         launch_frames(stream, tab_len[T_KEYS], d_desc, d_state, tab[T_KEYS], node4, child, slotoff, ccount, kbase,
                       lpos, lmeta, bframe);
-        launch_scatter(stream, tab_len[bt], d_desc, d_state, tab[bt], node4, kbase, sorted, forder, ccount, sorder,
+        launch_scatter(stream, tab_len[bt], d_desc, d_state, tab[bt], ppos, kbase, sorted, forder, ccount, sorder,
                        lpos, lmeta, fblist, stride, exclude);
         launch_frame_build(stream, tab_len[T_FRAMES], d_desc, d_state, tab[T_FRAMES], node4, child, cparent, cpend,
                            slotoff, cdata, lpos, lmeta, bframe, fblist, ph == kBuildPhases - 1);
