@@ -21,7 +21,7 @@ static constexpr int kBoundsPerBlock = 2048;
 // Bucketed phases: blocks per system of the frame packing (over the slots of the tree so
 // far) and of the frame builds (one workgroup per frame in flight; frames hold up to 512 bodies)
 static constexpr int kKeysBodiesPerBlock = 4096;
-static constexpr int kFramesBodiesPerBlock = 512;
+static constexpr int kFramesBodiesPerBlock = 256;
 
 #define NB_HIP(call)                                                                   \
     do {                                                                               \

@@ -226,6 +226,7 @@ __global__ __launch_bounds__(BLK) void k_build(const SysDesc* __restrict__ sd, S
     const int nq = list ? min(max(ld_c(&S->nfb), 0), n)
                         : (exclude > 1 ? n - (n + exclude - 1) / exclude : (exclude == 1 ? 0 : n));
     const int* blist = list ? list + d.body_off : nullptr;
+    if (queued && nq == 0) return;  // nothing queued (the usual case of the bucketed phases' fallback)
     int* C = child + 8 * (size_t)d.cell_off;
     int* par = cparent + d.cell_off;
     int* pend = cpend + d.cell_off;
