@@ -39,6 +39,26 @@ def dist(kind, n, seed):
     if kind == "cube":
         pos = r.uniform(-1, 1, size=(n, 3)).astype(np.float32)
         return pos, np.zeros_like(pos), np.full(n, 1.0 / max(n, 1), np.float32)
+    if kind == "clumps":
+        # Uniform cube with three clumps chosen by body index against the build's phases
+        # (every 4096th body, every 64th, the rest): 700 bodies of the strided phase in one
+        # 1e-3 cube and 1500 bodies of the last phase in another, each far more than a frame
+        # holds (buckets of about 710 and 1480 bodies: the queued fallback insertion), and
+        # 120 pairs 2e-6 apart in a third cube, a bucket that fits a frame but whose chains
+        # of cells overflow the frame's LDS cell pool (the whole frame falls back).
+        pos = r.uniform(-1, 1, size=(n, 3)).astype(np.float32)
+        idx = np.arange(n)
+        p1 = idx[(idx % 64 == 0) & (idx % 4096 != 0)][:700]
+        p2 = r.choice(idx[idx % 64 != 0], 1500, replace=False)
+        pos[p1] = (np.array([0.5, 0.5, 0.5]) + r.uniform(0, 1e-3, (len(p1), 3))).astype(np.float32)
+        pos[p2] = (np.array([-0.5, -0.5, 0.5]) + r.uniform(0, 1e-3, (len(p2), 3))).astype(np.float32)
+        p3 = r.choice(np.setdiff1d(idx[idx % 64 != 0], p2), 240, replace=False)
+        c = np.array([0.5, -0.5, -0.5]) + r.uniform(0, 1e-3, (120, 3))
+        d = r.normal(size=(120, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        pos[p3[:120]] = (c + 1e-6 * d).astype(np.float32)
+        pos[p3[120:]] = (c - 1e-6 * d).astype(np.float32)
+        return pos, np.zeros_like(pos), np.full(n, 1.0 / n, np.float32)
     raise ValueError(kind)
 
 
@@ -71,7 +91,8 @@ def compare_tree(s, t, n):
 
 @pytest.mark.parametrize("rules", [0, 1])
 @pytest.mark.parametrize("kind,n", [("normal", 1), ("normal", 2), ("normal", 63), ("normal", 64), ("normal", 65),
-                                    ("plummer", 1000), ("clusters", 3000), ("cube", 4096), ("plummer", 65536)])
+                                    ("plummer", 1000), ("clusters", 3000), ("cube", 4096), ("plummer", 65536),
+                                    ("clumps", 70000)])
 def test_tree_and_force_parity(orc, gpu, rules, kind, n):
     say(f"case rules={rules} kind={kind} n={n}")
     pos, vel, mass = dist(kind, n, 100 + n)
