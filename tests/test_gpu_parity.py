@@ -409,3 +409,27 @@ def test_largest_served_system_tree_exact(orc, gpu):
         compare_tree(s, t, n)
     finally:
         s.close()
+
+
+@pytest.mark.parametrize("rules", [0, 1])
+def test_four_million_tree_exact(orc, gpu, rules):
+    """N = 4,194,304 Plummer (seed 7): the first insertion phase (1024 bodies) exceeds the LDS
+    build's 512 and runs in global memory, then the two bucketed phases; bit-exact tree and
+    sort against the oracle, exact interaction counts and accelerations within tolerance on 32
+    sampled vote groups."""
+    n = 1 << 22
+    say(f"case 4M rules={rules}")
+    pos, vel, mass = ic.plummer(n, 7)
+    s, box = gpu_tree(pos, vel, mass, rules, theta=0.5, eps=0.01, G=1.0)
+    try:
+        t = orc.Tree(pos, mass, rules, cap=s.tree_info()["capacity"])
+        assert t.status == orc.OK
+        compare_tree(s, t, n)
+        groups = np.random.default_rng(1).choice(n // VG, 32, replace=False).astype(np.int32)
+        a_o, absacc, _ = t.force(0.5, 0.01, 1.0, vote=VG, groups=groups)
+        _, _, a_g = s.download()
+        members = t.force_order[(groups[:, None] * VG + np.arange(VG)[None, :]).reshape(-1)]
+        err = np.linalg.norm(a_g[members] - a_o[members], axis=1)
+        assert (err <= ACC_RTOL * absacc[members]).all()
+    finally:
+        s.close()
