@@ -810,6 +810,9 @@ __global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __
     float ax = 0.f, ay = 0.f, az = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
     int nint = 0;
     int* sk = stk[wave];
+    // byte offsets of a stack entry's cell in child (8 ints a cell) and cdata (8 float4)
+    auto slot_off = [](int e) { return (unsigned)(e >> 2) & ~31u; };
+    auto data_off = [](int e) { return (unsigned)e & ~127u; };
     int sp = 0;
     sk[0] = 0;
     sp = 1;
@@ -822,6 +825,10 @@ __global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __
         // itself the compiler sinks each child's load into the branch that uses it and
         // waits up to three times per cell, which lengthens the walk of the longest
         // waves.  The wait is inside the asm block: no load is in flight when it ends.
+        // The loads address the system's two arrays by a fixed 64-bit base plus a 32-bit
+        // byte offset in a scalar register (the cell pool is below 2^24 cells, so the
+        // offsets stay below 2^31), which keeps 64-bit address arithmetic and its
+        // registers out of the walk.
         // Cell 0 of the batch is visited first: with a pair, the second-popped cell.
         v8i sv[2];
         v16f q0[2], q1[2];
@@ -830,31 +837,31 @@ __global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __
         if (PAIR && sp > 0) {
             --sp;
             const int cur2 = __builtin_amdgcn_readfirstlane(sk[sp]);
-            const int na = cur >> 7, nb2 = cur2 >> 7;
             lv[0] = min(cur2 & 127, kMaxLevel);
             lv[1] = min(cur & 127, kMaxLevel);
             cells = 2;
             asm volatile(
-                "s_load_dwordx8 %0, %6, 0x0\n\t"
-                "s_load_dwordx16 %1, %7, 0x0\n\t"
-                "s_load_dwordx16 %2, %7, 0x40\n\t"
-                "s_load_dwordx8 %3, %8, 0x0\n\t"
-                "s_load_dwordx16 %4, %9, 0x0\n\t"
-                "s_load_dwordx16 %5, %9, 0x40\n\t"
+                "s_load_dwordx8 %[s0], %[cb], %[co0]\n\t"
+                "s_load_dwordx16 %[a0], %[db], %[do0]\n\t"
+                "s_load_dwordx16 %[b0], %[db], %[do0] offset:0x40\n\t"
+                "s_load_dwordx8 %[s1], %[cb], %[co1]\n\t"
+                "s_load_dwordx16 %[a1], %[db], %[do1]\n\t"
+                "s_load_dwordx16 %[b1], %[db], %[do1] offset:0x40\n\t"
                 "s_waitcnt lgkmcnt(0)"
-                : "=&s"(sv[0]), "=&s"(q0[0]), "=&s"(q1[0]), "=&s"(sv[1]), "=&s"(q0[1]), "=&s"(q1[1])
-                : "s"(C + 8 * nb2), "s"(CD + 8 * nb2), "s"(C + 8 * na), "s"(CD + 8 * na)
+                : [s0] "=&s"(sv[0]), [a0] "=&s"(q0[0]), [b0] "=&s"(q1[0]), [s1] "=&s"(sv[1]),
+                  [a1] "=&s"(q0[1]), [b1] "=&s"(q1[1])
+                : [cb] "s"(C), [db] "s"(CD), [co0] "s"(slot_off(cur2)), [do0] "s"(data_off(cur2)),
+                  [co1] "s"(slot_off(cur)), [do1] "s"(data_off(cur))
                 : "memory");
         } else {
-            const int na = cur >> 7;
             lv[0] = min(cur & 127, kMaxLevel);
             asm volatile(
-                "s_load_dwordx8 %0, %3, 0x0\n\t"
-                "s_load_dwordx16 %1, %4, 0x0\n\t"
-                "s_load_dwordx16 %2, %4, 0x40\n\t"
+                "s_load_dwordx8 %[s0], %[cb], %[co0]\n\t"
+                "s_load_dwordx16 %[a0], %[db], %[do0]\n\t"
+                "s_load_dwordx16 %[b0], %[db], %[do0] offset:0x40\n\t"
                 "s_waitcnt lgkmcnt(0)"
-                : "=&s"(sv[0]), "=&s"(q0[0]), "=&s"(q1[0])
-                : "s"(C + 8 * na), "s"(CD + 8 * na)
+                : [s0] "=&s"(sv[0]), [a0] "=&s"(q0[0]), [b0] "=&s"(q1[0])
+                : [cb] "s"(C), [db] "s"(CD), [co0] "s"(slot_off(cur)), [do0] "s"(data_off(cur))
                 : "memory");
         }
 #pragma unroll
