@@ -802,7 +802,10 @@ __global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __
     const int idx = first + lane;
     const bool valid = idx < w.end;
     const int b = min(max(sorted[d.body_off + (valid ? idx : w.end - 1)], 0), d.n - 1);
-    const float4 p = node4[d.node_off + b];
+    float4 p = node4[d.node_off + b];
+    // The empty asm consumes the position once, before the walk, so that the load's wait
+    // happens there; else the compiler waits again in every child test of every cell.
+    asm volatile("" : "+v"(p.x), "+v"(p.y), "+v"(p.z));
     const int* __restrict__ C = child + 8 * (size_t)d.cell_off;
     const float4* __restrict__ CD = cdata + 8 * (size_t)d.cell_off;
     const int n = d.n;
@@ -830,8 +833,12 @@ __global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __
         // offsets stay below 2^31), which keeps 64-bit address arithmetic and its
         // registers out of the walk.
         // Cell 0 of the batch is visited first: with a pair, the second-popped cell.
+        // The cells' opening thresholds are read from LDS before the batch; the empty asm
+        // after it consumes them, so that their wait (already met by the batch's) comes
+        // once, not before every child test.
         v8i sv[2];
         v16f q0[2], q1[2];
+        float dqv[2];
         int lv[2];
         int cells = 1;
         if (PAIR && sp > 0) {
@@ -839,6 +846,8 @@ __global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __
             const int cur2 = __builtin_amdgcn_readfirstlane(sk[sp]);
             lv[0] = min(cur2 & 127, kMaxLevel);
             lv[1] = min(cur & 127, kMaxLevel);
+            dqv[0] = dqs[lv[0]];
+            dqv[1] = dqs[lv[1]];
             cells = 2;
             asm volatile(
                 "s_load_dwordx8 %[s0], %[cb], %[co0]\n\t"
@@ -853,8 +862,10 @@ __global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __
                 : [cb] "s"(C), [db] "s"(CD), [co0] "s"(slot_off(cur2)), [do0] "s"(data_off(cur2)),
                   [co1] "s"(slot_off(cur)), [do1] "s"(data_off(cur))
                 : "memory");
+            asm volatile("" : "+v"(dqv[0]), "+v"(dqv[1]));
         } else {
             lv[0] = min(cur & 127, kMaxLevel);
+            dqv[0] = dqs[lv[0]];
             asm volatile(
                 "s_load_dwordx8 %[s0], %[cb], %[co0]\n\t"
                 "s_load_dwordx16 %[a0], %[db], %[do0]\n\t"
@@ -863,12 +874,13 @@ __global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __
                 : [s0] "=&s"(sv[0]), [a0] "=&s"(q0[0]), [b0] "=&s"(q1[0])
                 : [cb] "s"(C), [db] "s"(CD), [co0] "s"(slot_off(cur)), [do0] "s"(data_off(cur))
                 : "memory");
+            asm volatile("" : "+v"(dqv[0]));
         }
 #pragma unroll
         for (int c = 0; c < (PAIR ? 2 : 1); ++c) {
             if (c >= cells) break;
             const int lvl = lv[c];
-            const float dql = dqs[lvl];
+            const float dql = dqv[c];
             const int vs[8] = {sv[c][0], sv[c][1], sv[c][2], sv[c][3], sv[c][4], sv[c][5], sv[c][6], sv[c][7]};
             float4 qs[8];
             qs[0] = make_float4(q0[c][0], q0[c][1], q0[c][2], q0[c][3]);
