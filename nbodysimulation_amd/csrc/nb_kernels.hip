@@ -789,12 +789,14 @@ __global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __
                                                   float4* __restrict__ acc4, int* __restrict__ sorder) {
     constexpr int kS = PAIR ? kStackPair : kStack;
     __shared__ int stk[kForceWaves][kS];
-    __shared__ float dqs[kDqLen];
+    // the thresholds by level, over every value of a stack entry's 7-bit level field (the
+    // levels past kMaxLevel, which a built tree does not have, repeat the last threshold)
+    __shared__ float dqs[128];
     const BlkWork w = work[blockIdx.x];
     const SysDesc d = sd[w.sys];
     const SysState* S = st + w.sys;
     if (S->status || (d.rules == 0) != PAIR) return;  // the other instance's system
-    for (int i = threadIdx.x; i < kDqLen; i += blockDim.x) dqs[i] = S->dq[i];
+    for (int i = threadIdx.x; i < 128; i += blockDim.x) dqs[i] = S->dq[min(i, kMaxLevel)];
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int first = w.begin + wave * 64;
@@ -814,8 +816,8 @@ __global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __
     int nint = 0;
     int* sk = stk[wave];
     // byte offsets of a stack entry's cell in child (8 ints a cell) and cdata (8 float4)
-    auto slot_off = [](int e) { return (unsigned)(e >> 2) & ~31u; };
     auto data_off = [](int e) { return (unsigned)e & ~127u; };
+    auto slot_off = [&](int e) { return data_off(e) >> 2; };
     int sp = 0;
     sk[0] = 0;
     sp = 1;
@@ -844,8 +846,8 @@ __global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __
         if (PAIR && sp > 0) {
             --sp;
             const int cur2 = __builtin_amdgcn_readfirstlane(sk[sp]);
-            lv[0] = min(cur2 & 127, kMaxLevel);
-            lv[1] = min(cur & 127, kMaxLevel);
+            lv[0] = cur2 & 127;
+            lv[1] = cur & 127;
             dqv[0] = dqs[lv[0]];
             dqv[1] = dqs[lv[1]];
             cells = 2;
@@ -864,7 +866,7 @@ __global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __
                 : "memory");
             asm volatile("" : "+v"(dqv[0]), "+v"(dqv[1]));
         } else {
-            lv[0] = min(cur & 127, kMaxLevel);
+            lv[0] = cur & 127;
             dqv[0] = dqs[lv[0]];
             asm volatile(
                 "s_load_dwordx8 %[s0], %[cb], %[co0]\n\t"
@@ -879,7 +881,8 @@ __global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __
 #pragma unroll
         for (int c = 0; c < (PAIR ? 2 : 1); ++c) {
             if (c >= cells) break;
-            const int lvl = lv[c];
+            // a pushed child's entry, ((v - n) << 7) | (level + 1), as (v << 7) + pb (mod 2^32)
+            const unsigned pb = (unsigned)(lv[c] + 1) - ((unsigned)n << 7);
             const float dql = dqv[c];
             const int vs[8] = {sv[c][0], sv[c][1], sv[c][2], sv[c][3], sv[c][4], sv[c][5], sv[c][6], sv[c][7]};
             float4 qs[8];
@@ -914,7 +917,7 @@ __global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __
                     }
                     ++nint;
                 } else if (sp < kS) {  // the bounds above (one-cell walk: 7 * 64 + 8) keep sp below kS
-                    sk[sp] = ((v - n) << 7) | (lvl + 1);
+                    sk[sp] = (int)(((unsigned)v << 7) + pb);
                     ++sp;
                 }
             }
