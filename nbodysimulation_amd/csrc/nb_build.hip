@@ -307,8 +307,10 @@ struct FrameLds {
 };
 
 // One workgroup per frame (grid-stride over the system's frames): entries e = 4 * thread + t
-// in frame order, every bucket's subtree built at once, level by level.
-__global__ __launch_bounds__(kFrameThreads) void k_frame_build(
+// in frame order, every bucket's subtree built at once, level by level.  At most 128 VGPRs
+// (four waves per SIMD): left free the compiler takes 129, which allows three, fewer
+// workgroups per CU than the LDS allows (eight).
+__global__ __launch_bounds__(kFrameThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_frame_build(
     const SysDesc* __restrict__ sd, SysState* st, const BlkWork* __restrict__ work, const float4* __restrict__ node4,
     int* child, int* cparent, int* cpend, int* slotoff, float4* cdata, const float4* __restrict__ lpos,
     const int4* __restrict__ lmeta, const int2* __restrict__ bframe, int* fblist, int last) {
