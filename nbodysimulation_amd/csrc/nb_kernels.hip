@@ -900,8 +900,11 @@ __global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __
                 if (v < 0) continue;
                 const float dx = qs[k].x - p.x, dy = qs[k].y - p.y, dz = qs[k].z - p.z;
                 const float d2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, e2)));
+                // the vote: every lane far enough, written as "no lane near" (the same for
+                // the finite d2 of a pass over finite positions), whose compare result
+                // the branch tests directly, without a scalar compare against the exec mask
                 bool take = v < n;
-                if (!take) take = __all(d2 >= dql);
+                if (!take) take = !__any(d2 < dql);
                 if (take) {
                     const float ri = __builtin_amdgcn_rsqf(d2);
                     const float F = qs[k].w * (ri * ri * ri);
