@@ -86,11 +86,11 @@ __global__ __launch_bounds__(256) void k_classify(const SysDesc* __restrict__ sd
     const SysDesc d = sd[w.sys];
     SysState* S = st + w.sys;
     if (S->status || d.algo != 0) return;
-    // the phase's counters start at zero (the kernels that used them last have completed)
+    // the phase's counters start at zero (the kernels that used them last have completed);
+    // the fallback list (nfb) collects every phase's bodies until the insertion after the last
     if (w.begin == 0 && threadIdx.x == 0) {
         S->nframes = 0;
         S->listpos = 0;
-        S->nfb = 0;
         S->bq = 0;
     }
     const int n = d.n;

@@ -25,9 +25,10 @@ void launch_tree_clear(hipStream_t s, int nblk, const SysDesc* sd, const SysStat
                        int* child, int* cpend);
 // slots NULL: 0..nsys-1; sd and slotoff NULL: the roots' per-slot counts are left alone
 void launch_tree_init(hipStream_t s, int nsys, const int* slots, const SysDesc* sd, SysState* st, int* slotoff);
+// fblist: the build's fallback list, which takes the phase's bodies when the LDS tree overflows
 void launch_build_top(hipStream_t s, int nsys, const int* slots, const SysDesc* sd, SysState* st,
                       const float4* node4, int* child, int* cparent, int* cpend, float4* cdata, int* slotoff,
-                      int stride);
+                      int stride, int* fblist);
 // list: the queued phase takes its bodies from this per-system list (SysState::nfb entries)
 void launch_build(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
                   const float4* node4, int* child, int* cparent, int* cpend, float4* cdata, int* slotoff, int stride,
@@ -138,6 +139,7 @@ struct Engine {
     std::vector<Slot> active;
     bool any_bh = false, any_bf = false;
     bool any_bh_ref = false, any_bh_std = false;  // Barnes-Hut systems under REFERENCE / STANDARD rules
+    bool any_top_global = false;  // a Barnes-Hut system whose first insertion phase exceeds kTopBodies
     int bf_tb = 1;  // brute-force targets per lane of the current batch (1 or 2)
     int law = 1;
     bool kahan = false;

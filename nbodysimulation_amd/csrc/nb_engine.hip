@@ -209,7 +209,7 @@ nb_status Engine::activate(const std::vector<Slot>& slots) {
     std::vector<char> seen(capc.systems, 0);
     long long bf_waves2 = 0;  // brute-force waves of the batch at two targets per lane
     any_bh = any_bf = any_collision = false;
-    any_bh_ref = any_bh_std = false;
+    any_bh_ref = any_bh_std = any_top_global = false;
     long long bh_bodies = 0;  // the batch's Barnes-Hut bodies: the queued build phase shares its budget by them
     for (const Slot& sl : slots)
         if (sl.spec.algo == NB_BARNES_HUT && sl.spec.n > 0) bh_bodies += sl.spec.n;
@@ -254,6 +254,7 @@ nb_status Engine::activate(const std::vector<Slot>& slots) {
         if (bh && d.n > 0) {
             any_bh = true;
             (d.rules == 0 ? any_bh_ref : any_bh_std) = true;
+            if ((d.n + kPhaseStride[0] - 1) / kPhaseStride[0] > kTopBodies) any_top_global = true;
             d.bbox_first = (int)t[T_BOUNDS].size();
             add_ranges(t[T_BOUNDS], s, d.n, kBoundsPerBlock);
             d.bbox_blocks = (int)t[T_BOUNDS].size() - d.bbox_first;
@@ -375,16 +376,17 @@ void Engine::tree_reset() {
     launch_tree_clear(stream, tab_len[T_CELLS], d_desc, d_state, tab[T_CELLS], child, cpend);
     launch_tree_init(stream, nactive, d_slots, d_desc, d_state, slotoff);
 }
-// The first phase (every 4096th body) in LDS (k_build_top), or by insertion when it does not
-// fit there; each later phase bucketed (nb_build.hip) against the tree of the phases before
-// it, with the queued insertion taking the bodies that the bucketed kernels leave (buckets
-// over 256 bodies, subtrees over the LDS pool or at the depth limit).
+// The first phase (every 4096th body) in LDS (k_build_top), or by insertion in global memory
+// for a system with more first-phase bodies than the LDS build takes; each later phase
+// bucketed (nb_build.hip) against the tree of the phases before it; then the queued insertion
+// of the bodies that any phase left (see below).
 void Engine::build() {
     TraceRange tr("nb:build");
     launch_build_top(stream, nactive, d_slots, d_desc, d_state, node4, child, cparent, cpend, cdata, slotoff,
-                     kPhaseStride[0]);
-    launch_build(stream, tab_len[T_PHASE0], d_desc, d_state, tab[T_PHASE0], node4, child, cparent, cpend, cdata,
-                 slotoff, kPhaseStride[0], 0, nullptr);
+                     kPhaseStride[0], fblist);
+    if (any_top_global)  // only systems over kTopBodies first-phase bodies (S->top 0) take part
+        launch_build(stream, tab_len[T_PHASE0], d_desc, d_state, tab[T_PHASE0], node4, child, cparent, cpend,
+                     cdata, slotoff, kPhaseStride[0], 0, nullptr);
     for (int ph = 1; ph < kBuildPhases; ++ph) {
         // the phase's bodies: a strided table, or every body (skipping the earlier phases')
         const bool strided = kPhaseStride[ph] > 1;
@@ -398,9 +400,12 @@ void Engine::build() {
                        lpos, lmeta, fblist, stride, exclude);
         launch_frame_build(stream, tab_len[T_FRAMES], d_desc, d_state, tab[T_FRAMES], node4, child, cparent, cpend,
                            slotoff, cdata, lpos, lmeta, bframe, fblist, ph == kBuildPhases - 1);
-        launch_build(stream, tab_len[T_PHASE0 + kBuildPhases - 1], d_desc, d_state,
-                     tab[T_PHASE0 + kBuildPhases - 1], node4, child, cparent, cpend, cdata, slotoff, 1, 0, fblist);
     }
+    // the bodies every phase left (the fallback list: a frame over the LDS pool or the depth
+    // limit, a bucket over the frame budget, a first phase over the LDS cell pool), inserted
+    // last; usually none, and the launch returns at once
+    launch_build(stream, tab_len[T_PHASE0 + kBuildPhases - 1], d_desc, d_state, tab[T_PHASE0 + kBuildPhases - 1],
+                 node4, child, cparent, cpend, cdata, slotoff, 1, 0, fblist);
 }
 void Engine::summarize() {
     TraceRange tr("nb:summarize");

@@ -30,6 +30,7 @@
 namespace nb {
 
 constexpr int kMaxLevel = 64;          // deepest cell level (root = 0)
+constexpr int kTopBodies = 512;        // most bodies of a first insertion phase built in LDS (k_build_top)
 constexpr int kDqLen = kMaxLevel + 1;  // opening thresholds per level
 constexpr int kBfWaves = 16;    // brute-force workgroup: 16 waves of 64
 // Build workgroups: the queued last phase shares one allocation atomic per round among
@@ -92,7 +93,8 @@ struct SysState {
     int step;
     float dq[kDqLen];
     int order_ok;   // 1: sorder holds a permutation of the bodies (the last full force pass's order)
-    int top;        // 1: this step's first insertion phase was built in LDS (k_build_top)
+    int top;        // this step's first insertion phase: 1 built in LDS (k_build_top); 2 its bodies
+                    // went to the fallback list (the LDS cell pool overflowed); 0 neither
     int pad1;
     // a bucketed build phase: frames and frame-list entries claimed (the two halves of one
     // 8-byte-aligned word, claimed together), and bodies sent to the fallback insertion

@@ -167,6 +167,7 @@ __global__ void k_tree_init(int nsys, const int* __restrict__ slots, const SysDe
     st[s].max_depth = 1;  // build_tree.cl:28
     st[s].bq = 0;
     st[s].top = 0;
+    st[s].nfb = 0;  // the build's fallback list: its phases append, its queued insertion empties it
 }
 
 // ---------------------------------------------------------------------------
@@ -363,8 +364,10 @@ __global__ __launch_bounds__(BLK) void k_build(const SysDesc* __restrict__ sd, S
             }
             s_base = base;
             if (queued) {
-                s_qbase = qb;
-                drained = drained || s_qbase >= nq;
+                s_qbase = qb;  // without a claim (no lane wanted a body): nq, so no entry is taken
+                // only a claim that ran can find the queue empty: a round in which every lane
+                // was still busy claims nothing and must not stop the workgroup's later claims
+                if (claim) drained = qb >= nq;
             }
         }
         __syncthreads();
@@ -440,16 +443,20 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 // body holding a contested slot allocates a new cell there; after another barrier the
 // contested bodies move into it.  So a cell exists exactly for every path prefix that two
 // or more of the bodies share, which is the topology k_build's insertion gives.  Cell
-// numbers are those of the LDS tree.  More than kTopBodies bodies, or more than kTopCells
-// cells, writes nothing: S->top stays 0 and k_build runs the phase in global memory.
+// numbers are those of the LDS tree.  More than kTopBodies bodies writes nothing (S->top stays
+// 0, and k_build runs the phase in global memory: the host launches it for such systems); more
+// than kTopCells cells writes no cell and appends the phase's bodies to the build's fallback
+// list (S->top = 2), which the queued insertion after the last phase takes: the tree's
+// topology does not depend on the order of insertion.
 
 constexpr int kTopCells = 1280;  // 50 KiB of LDS: 8 slots, a parent link and contested bits per cell
-constexpr int kTopBodies = 512;  // 8 KiB of LDS: the phase's bodies
+static_assert(kTopBodies == 512, "k_build_top: 8 KiB of LDS for the phase's bodies, one lane each");
 
 __global__ __launch_bounds__(kTopBodies) void k_build_top(int nsys, const int* __restrict__ slots,
                                                           const SysDesc* __restrict__ sd, SysState* st,
                                                           const float4* __restrict__ node4, int* child, int* cparent,
-                                                          int* cpend, float4* cdata, int* slotoff, int stride) {
+                                                          int* cpend, float4* cdata, int* slotoff, int stride,
+                                                          int* fblist) {
     __shared__ int sC[8 * kTopCells];
     __shared__ int sPar[kTopCells];
     __shared__ int sCont[kTopCells];  // bit k: slot k was claimed by more than one body this level
@@ -519,7 +526,15 @@ __global__ __launch_bounds__(kTopBodies) void k_build_top(int nsys, const int* _
     depth = wave_max_i(depth);
     if ((threadIdx.x & 63) == 0) atomicMax(&sDepth, depth);
     __syncthreads();
-    if (sFail) return;
+    if (sFail) {  // uniform; with a depth overflow the status stops the step
+        if (threadIdx.x == 0) {
+            sN = atomicAdd(&S->nfb, nb);  // the list's room: n entries (each body once per step)
+            S->top = 2;
+        }
+        __syncthreads();
+        if (j < nb && sN + j < n) fblist[d.body_off + sN + j] = b;
+        return;
+    }
     const int nc = sN;
     int* C = child + 8 * (size_t)d.cell_off;
     for (int c = threadIdx.x; c < nc; c += blockDim.x) {
@@ -1388,10 +1403,10 @@ void launch_tree_init(hipStream_t s, int nsys, const int* slots, const SysDesc* 
 }
 void launch_build_top(hipStream_t s, int nsys, const int* slots, const SysDesc* sd, SysState* st,
                       const float4* node4, int* child, int* cparent, int* cpend, float4* cdata, int* slotoff,
-                      int stride) {
+                      int stride, int* fblist) {
     if (nsys <= 0) return;
     NB_LAUNCH(k_build_top, nsys, kTopBodies, s, nsys, slots, sd, st, node4, child, cparent, cpend, cdata, slotoff,
-              stride);
+              stride, fblist);
 }
 void launch_build(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
                   const float4* node4, int* child, int* cparent, int* cpend, float4* cdata, int* slotoff, int stride,
