@@ -480,11 +480,15 @@ static void enqueue_force(nb_sim* sim, hipEvent_t* e) {
         if (e) (void)hipEventRecord(e[i], g.stream);
     };
     if (is_bh(sim)) {
-        mark(0);
-        g.bounds();
-        mark(1);
-        g.tree_reset();
-        mark(2);
+        if (e) {  // per-stage events: the stages launched one by one
+            mark(0);
+            g.bounds();
+            mark(1);
+            g.tree_reset();
+            mark(2);
+        } else {
+            g.bounds_reset();
+        }
         g.build();
         mark(3);
         g.summarize();

@@ -23,6 +23,11 @@ void launch_bounds(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, con
                    const float4* node4, float* partial);
 void launch_tree_clear(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
                        int* child, int* cpend);
+// the bounds pass (nbounds workgroups of table wb) and the tree clear (ncells of table wc) in
+// one launch; the root's re-seeding is left to k_build_top
+void launch_bounds_clear(hipStream_t s, int nbounds, const BlkWork* wb, int ncells, const BlkWork* wc,
+                         const SysDesc* sd, SysState* st, const float4* node4, float* partial, int* child,
+                         int* cpend);
 // slots NULL: 0..nsys-1; sd and slotoff NULL: the roots' per-slot counts are left alone
 void launch_tree_init(hipStream_t s, int nsys, const int* slots, const SysDesc* sd, SysState* st, int* slotoff);
 // fblist: the build's fallback list, which takes the phase's bodies when the LDS tree overflows
@@ -188,6 +193,9 @@ struct Engine {
     // whole-batch stage launches (asynchronous on `stream`)
     void bounds();
     void tree_reset();
+    // bounds() and tree_reset() of a step in one launch, the root's re-seeding done by
+    // build()'s first kernel: only where build() follows at once
+    void bounds_reset();
     void build();
     void summarize();
     void sort();

@@ -371,6 +371,11 @@ void Engine::bounds() {
     TraceRange tr("nb:bounds");
     launch_bounds(stream, tab_len[T_BOUNDS], d_desc, d_state, tab[T_BOUNDS], node4, d_partial);
 }
+void Engine::bounds_reset() {
+    TraceRange tr("nb:bounds_reset");
+    launch_bounds_clear(stream, tab_len[T_BOUNDS], tab[T_BOUNDS], tab_len[T_CELLS], tab[T_CELLS], d_desc, d_state,
+                        node4, d_partial, child, cpend);
+}
 void Engine::tree_reset() {
     TraceRange tr("nb:tree_reset");
     launch_tree_clear(stream, tab_len[T_CELLS], d_desc, d_state, tab[T_CELLS], child, cpend);
@@ -457,8 +462,7 @@ void Engine::brute(hipStream_t on) {
 void Engine::force_all() {
     const bool beside = any_bh && any_bf && side;
     if (any_bh) {
-        bounds();
-        tree_reset();
+        bounds_reset();
         build();
     }
     if (beside) {

@@ -40,12 +40,11 @@ namespace nb {
 // workgroup reduces its range (DPP inside a wave, LDS across waves), the last
 // workgroup of a system (ticket) reduces the partials.
 
-__global__ __launch_bounds__(256) void k_bounds(const SysDesc* __restrict__ sd, SysState* st,
-                                                const BlkWork* __restrict__ work,
-                                                const float4* __restrict__ node4, float* partial) {
+// One workgroup of the bounds pass (256 lanes): w its range, blk its place in the bounds table.
+__device__ __forceinline__ void bounds_block(const SysDesc* __restrict__ sd, SysState* st, const BlkWork w, int blk,
+                                             const float4* __restrict__ node4, float* partial) {
     __shared__ float red[4][7];
     __shared__ int is_last;
-    const BlkWork w = work[blockIdx.x];
     const SysDesc d = sd[w.sys];
     SysState* S = st + w.sys;
     const float4* nodes = node4 + d.node_off;
@@ -77,7 +76,7 @@ __global__ __launch_bounds__(256) void k_bounds(const SysDesc* __restrict__ sd, 
             for (int k = 0; k < 3; ++k) r[k] = fminf(r[k], red[q][k]);
             for (int k = 3; k < 7; ++k) r[k] = fmaxf(r[k], red[q][k]);
         }
-        for (int k = 0; k < 7; ++k) st_cf(partial + 7 * (size_t)blockIdx.x + k, r[k]);
+        for (int k = 0; k < 7; ++k) st_cf(partial + 7 * (size_t)blk + k, r[k]);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_s_waitcnt(0);
         const int t = __hip_atomic_fetch_add(&S->ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -135,13 +134,18 @@ __global__ __launch_bounds__(256) void k_bounds(const SysDesc* __restrict__ sd, 
     S->ticket = 0;
 }
 
+__global__ __launch_bounds__(256) void k_bounds(const SysDesc* __restrict__ sd, SysState* st,
+                                                const BlkWork* __restrict__ work,
+                                                const float4* __restrict__ node4, float* partial) {
+    bounds_block(sd, st, work[blockIdx.x], blockIdx.x, node4, partial);
+}
+
 // ---------------------------------------------------------------------------
 // reset: clear the slots of the cells the previous build used (so every unused
 // cell holds -1 slots), then re-seed the root.
 
-__global__ __launch_bounds__(256) void k_tree_clear(const SysDesc* __restrict__ sd, const SysState* __restrict__ st,
-                                                    const BlkWork* __restrict__ work, int* child, int* cpend) {
-    const BlkWork w = work[blockIdx.x];
+__device__ __forceinline__ void tree_clear_block(const SysDesc* __restrict__ sd, const SysState* __restrict__ st,
+                                                 const BlkWork w, int* child, int* cpend) {
     const SysDesc d = sd[w.sys];
     const int used = min(st[w.sys].ncells, d.cap);
     const int ci = w.begin + threadIdx.x;
@@ -151,6 +155,25 @@ __global__ __launch_bounds__(256) void k_tree_clear(const SysDesc* __restrict__ 
     c4[0] = m1;
     c4[1] = m1;
     cpend[d.cell_off + ci] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_tree_clear(const SysDesc* __restrict__ sd, const SysState* __restrict__ st,
+                                                    const BlkWork* __restrict__ work, int* child, int* cpend) {
+    tree_clear_block(sd, st, work[blockIdx.x], child, cpend);
+}
+
+// A step's bounds pass and tree clear in one launch: workgroups below nb take the bounds
+// table (their ticket counts only them), the others the cell table; the root's re-seeding
+// (k_tree_init's work) is then done by k_build_top, the build's first launch.
+__global__ __launch_bounds__(256) void k_bounds_clear(const SysDesc* __restrict__ sd, SysState* st,
+                                                      const BlkWork* __restrict__ wb, int nb,
+                                                      const BlkWork* __restrict__ wc,
+                                                      const float4* __restrict__ node4, float* partial, int* child,
+                                                      int* cpend) {
+    if ((int)blockIdx.x < nb)
+        bounds_block(sd, st, wb[blockIdx.x], blockIdx.x, node4, partial);
+    else
+        tree_clear_block(sd, st, wc[blockIdx.x - nb], child, cpend);
 }
 
 __global__ void k_tree_init(int nsys, const int* __restrict__ slots, const SysDesc* __restrict__ sd, SysState* st,
@@ -468,6 +491,20 @@ __global__ __launch_bounds__(kTopBodies) void k_build_top(int nsys, const int* _
     SysState* S = st + sys;
     const int n = d.n;
     const int nb = (n + stride - 1) / stride;  // bodies j * stride, j < nb
+    // the build starts from the root alone (as k_tree_init leaves it; a step's fused bounds
+    // and clear launch leaves the re-seeding to this kernel)
+    if (threadIdx.x == 0) {
+        S->ncells = 1;
+        S->max_depth = 1;  // build_tree.cl:28
+        S->bq = 0;
+        S->top = 0;
+        S->nfb = 0;
+        if (d.algo == 0 && d.cap > 0) {
+            int4* so = reinterpret_cast<int4*>(slotoff + 8 * (size_t)d.cell_off);
+            so[0] = make_int4(0, 0, 0, 0);
+            so[1] = make_int4(0, 0, 0, 0);
+        }
+    }
     if (d.algo != 0 || n == 0 || nb > kTopBodies || S->status) return;
     for (int i = threadIdx.x; i < 8 * kTopCells; i += blockDim.x) sC[i] = kEmpty;
     for (int i = threadIdx.x; i < kTopCells; i += blockDim.x) sCont[i] = 0;
@@ -1396,6 +1433,11 @@ void launch_bounds(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, con
 void launch_tree_clear(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
                        int* child, int* cpend) {
     NB_LAUNCH(k_tree_clear, nblk, 256, s, sd, st, work, child, cpend);
+}
+void launch_bounds_clear(hipStream_t s, int nbounds, const BlkWork* wb, int ncells, const BlkWork* wc,
+                         const SysDesc* sd, SysState* st, const float4* node4, float* partial, int* child,
+                         int* cpend) {
+    NB_LAUNCH(k_bounds_clear, nbounds + ncells, 256, s, sd, st, wb, nbounds, wc, node4, partial, child, cpend);
 }
 void launch_tree_init(hipStream_t s, int nsys, const int* slots, const SysDesc* sd, SysState* st, int* slotoff) {
     if (nsys <= 0) return;
