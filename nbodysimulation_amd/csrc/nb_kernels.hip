@@ -4,6 +4,8 @@
 //   k_bounds      hybrid_reduce_root + parallel_reduce_root  kernels/boundary_reduce.cl:6-159
 //   k_tree_clear  init (child reset)                         kernels/init.cl:2-36
 //   k_tree_init   init (root, bottom)                        kernels/init.cl:22-34
+//   k_bounds_clear  k_bounds + k_tree_clear in one launch of a step (the root's re-seeding
+//                 then done by k_build_top)
 //   k_build       build_tree                                 kernels/build_tree.cl:6-165
 //   k_summarize   saturate_tree                              kernels/saturate_tree.cl:6-146
 //   k_sort        sort (empty in the reference)              kernels/sort.cl
