@@ -25,9 +25,11 @@ void launch_tree_clear(hipStream_t s, int nblk, const SysDesc* sd, const SysStat
                        int* child, int* cpend);
 // the bounds pass (nbounds workgroups of table wb) and the tree clear (ncells of table wc) in
 // one launch; the root's re-seeding is left to k_build_top
+// og/olen/keys/oout: the force pass's dispatch order (as launch_force_order) computed by one
+// more workgroup when olen is in (0, kOrderSmall]; otherwise not computed
 void launch_bounds_clear(hipStream_t s, int nbounds, const BlkWork* wb, int ncells, const BlkWork* wc,
                          const SysDesc* sd, SysState* st, const float4* node4, float* partial, int* child,
-                         int* cpend);
+                         int* cpend, const BlkWork* og, int olen, int* keys, BlkWork* oout);
 // slots NULL: 0..nsys-1; sd and slotoff NULL: the roots' per-slot counts are left alone
 void launch_tree_init(hipStream_t s, int nsys, const int* slots, const SysDesc* sd, SysState* st, int* slotoff);
 // fblist: the build's fallback list, which takes the phase's bodies when the LDS tree overflows
@@ -56,14 +58,13 @@ void launch_summarize(hipStream_t s, int nblk, const SysDesc* sd, const SysState
                       float4* node4, const int* child, const int* cparent, int* cpend, int* ccount, int* slotoff,
                       float4* cdata);
 void launch_sort(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work, const float4* node4,
-                 const int* child, const int* slotoff, int* sorted, int* forder, const float4* acc4, int* keys,
-                 const int* sorder);
+                 const int* child, const int* slotoff, int* sorted, int* forder, const int* sorder);
 void launch_force_order(hipStream_t s, const BlkWork* in, int len, int* keys, BlkWork* out);
 // pair: the instance for systems under REFERENCE rules (walks two cells per scalar-load
 // round trip); otherwise the one for STANDARD rules.  Each skips the other's systems.
 void launch_force_bh(hipStream_t s, int nblk, bool kahan, bool pair, const SysDesc* sd, SysState* st,
                      const BlkWork* work, const float4* node4, const int* child, const float4* cdata,
-                     const int* order, float4* acc4, int* sorder);
+                     const int* order, float4* acc4, int* sorder, int* keys);
 // mode >= 0: also integrate (k_update's mode 0 or 1); mode 0 writes the moved
 // positions to out4 (same layout as node4), never in place
 void launch_brute(hipStream_t s, int nblk, int law, bool kahan, int tb, const SysDesc* sd, const SysState* st,
@@ -164,7 +165,8 @@ struct Engine {
     SysState* d_state = nullptr;
     float* d_partial = nullptr;
     int* d_slots = nullptr;  // slots of the active systems (tree_init touches only these)
-    int* gkey = nullptr;     // per force workgroup: dispatch-order key (k_sort -> k_force_order)
+    int* gkey = nullptr;     // per force workgroup: dispatch-order key (k_force_bh -> the next pass's ordering)
+    bool order_done = false; // host: this step's bounds_reset() computed the force pass's dispatch order
     bool lpt = false;        // force workgroups dispatched longest first (few waves per SIMD)
     int nactive = 0;
 

@@ -321,6 +321,7 @@ nb_status Engine::activate(const std::vector<Slot>& slots) {
     NB_HIP(hipMemcpyAsync(d_desc, hdesc.data(), sizeof(SysDesc) * capc.systems, hipMemcpyHostToDevice, stream));
     if (!ids.empty()) NB_HIP(hipMemcpyAsync(d_slots, ids.data(), sizeof(int) * ids.size(), hipMemcpyHostToDevice, stream));
     NB_HIP(hipStreamSynchronize(stream));
+    order_done = false;
     nactive = (int)ids.size();
     active = slots;
     return NB_OK;
@@ -373,8 +374,11 @@ void Engine::bounds() {
 }
 void Engine::bounds_reset() {
     TraceRange tr("nb:bounds_reset");
+    const bool ord = lpt && tab_len[T_GROUPS] <= kOrderSmall;
     launch_bounds_clear(stream, tab_len[T_BOUNDS], tab[T_BOUNDS], tab_len[T_CELLS], tab[T_CELLS], d_desc, d_state,
-                        node4, d_partial, child, cpend);
+                        node4, d_partial, child, cpend, tab[T_GROUPS], ord ? tab_len[T_GROUPS] : 0, gkey,
+                        tab[T_FORDER]);
+    order_done = ord;
 }
 void Engine::tree_reset() {
     TraceRange tr("nb:tree_reset");
@@ -420,15 +424,19 @@ void Engine::summarize() {
 void Engine::sort() {
     TraceRange tr("nb:sort");
     launch_sort(stream, tab_len[T_BODY], d_desc, d_state, tab[T_BODY], node4, child, slotoff, sorted, forder,
-                acc4, lpt ? gkey : nullptr, sorder);
+                sorder);
 }
 void Engine::force() {
     TraceRange tr("nb:force");
-    if (lpt) launch_force_order(stream, tab[T_GROUPS], tab_len[T_GROUPS], gkey, tab[T_FORDER]);
+    // the dispatch order from the last full pass's keys: already computed by this step's
+    // bounds_reset(), or here
+    if (lpt && !order_done) launch_force_order(stream, tab[T_GROUPS], tab_len[T_GROUPS], gkey, tab[T_FORDER]);
+    order_done = false;
     for (int pair = 0; pair < 2; ++pair)
         if (pair ? any_bh_ref : any_bh_std)
             launch_force_bh(stream, tab_len[T_GROUPS], kahan, pair != 0, d_desc, d_state,
-                            tab[lpt ? T_FORDER : T_GROUPS], node4, child, cdata, forder, acc4, sorder);
+                            tab[lpt ? T_FORDER : T_GROUPS], node4, child, cdata, forder, acc4, sorder,
+                            lpt ? gkey : nullptr);
 }
 void Engine::force_range(int sys, int lo, int hi) {
     TraceRange tr("nb:force_range");
@@ -438,7 +446,7 @@ void Engine::force_range(int sys, int lo, int hi) {
     const int nblk = (stop - start + per - 1) / per;
     launch_range_table(stream, tab[T_RANGE], nblk, sys, start, stop, lo, hi, per);
     launch_force_bh(stream, nblk, kahan, hdesc[sys].rules == 0, d_desc, d_state, tab[T_RANGE], node4, child, cdata,
-                    forder, acc4, nullptr);
+                    forder, acc4, nullptr, nullptr);
 }
 void Engine::brute_range(int sys, int lo, int hi) {
     TraceRange tr("nb:brute_force_range");

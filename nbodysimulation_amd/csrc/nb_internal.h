@@ -48,6 +48,7 @@ constexpr int kBfTileStride = 4 * 17;  // a wave's LDS tile: up to 4 parts of 16
 #endif
 constexpr int kForceWaves = NB_FORCE_WAVES;  // Barnes-Hut force workgroup: waves of 64
 constexpr int kForceVote = 64;  // bodies per opening decision: one wave, one lane per body
+constexpr int kOrderSmall = 8192;  // force workgroups that one 256-lane workgroup orders (keys in registers)
 constexpr int kOrderMax = 32768;  // force workgroups that k_force_order can order (keys in LDS)
 constexpr int kBfFuseMax = 16384;  // single systems up to this size run the integrator inside k_brute
 constexpr int kEnergyLen = 9;   // kinetic, potential, p[3], L[3], mass

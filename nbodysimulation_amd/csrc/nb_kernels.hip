@@ -4,8 +4,8 @@
 //   k_bounds      hybrid_reduce_root + parallel_reduce_root  kernels/boundary_reduce.cl:6-159
 //   k_tree_clear  init (child reset)                         kernels/init.cl:2-36
 //   k_tree_init   init (root, bottom)                        kernels/init.cl:22-34
-//   k_bounds_clear  k_bounds + k_tree_clear in one launch of a step (the root's re-seeding
-//                 then done by k_build_top)
+//   k_bounds_clear  k_bounds + k_tree_clear (+ the force pass's dispatch order) in one launch
+//                 of a step (the root's re-seeding then done by k_build_top)
 //   k_build       build_tree                                 kernels/build_tree.cl:6-165
 //   k_summarize   saturate_tree                              kernels/saturate_tree.cl:6-146
 //   k_sort        sort (empty in the reference)              kernels/sort.cl
@@ -164,19 +164,6 @@ __global__ __launch_bounds__(256) void k_tree_clear(const SysDesc* __restrict__ 
     tree_clear_block(sd, st, work[blockIdx.x], child, cpend);
 }
 
-// A step's bounds pass and tree clear in one launch: workgroups below nb take the bounds
-// table (their ticket counts only them), the others the cell table; the root's re-seeding
-// (k_tree_init's work) is then done by k_build_top, the build's first launch.
-__global__ __launch_bounds__(256) void k_bounds_clear(const SysDesc* __restrict__ sd, SysState* st,
-                                                      const BlkWork* __restrict__ wb, int nb,
-                                                      const BlkWork* __restrict__ wc,
-                                                      const float4* __restrict__ node4, float* partial, int* child,
-                                                      int* cpend) {
-    if ((int)blockIdx.x < nb)
-        bounds_block(sd, st, wb[blockIdx.x], blockIdx.x, node4, partial);
-    else
-        tree_clear_block(sd, st, wc[blockIdx.x - nb], child, cpend);
-}
 
 __global__ void k_tree_init(int nsys, const int* __restrict__ slots, const SysDesc* __restrict__ sd, SysState* st,
                             int* slotoff) {
@@ -737,9 +724,6 @@ __global__ __launch_bounds__(256) void k_summarize(const SysDesc* __restrict__ s
 // re-derives its path (the same octant digits as the build) and adds them up:
 // no waiting, no dependence on the order in which cells were allocated.
 
-// keys (optional): per force workgroup of the coming force pass, the largest
-// interaction count that its waves' first bodies had in the previous force pass
-// (acc4.w), the dispatch-order key of k_force_order.
 // The same walk gives each body's place in the force order: at every cell, the
 // bodies of the children that come earlier on the cell's Hilbert order precede
 // it.  Both ranks come from the cell's 8 inclusive per-slot counts (one 32-byte
@@ -747,9 +731,7 @@ __global__ __launch_bounds__(256) void k_summarize(const SysDesc* __restrict__ s
 __global__ __launch_bounds__(256) void k_sort(const SysDesc* __restrict__ sd, SysState* st,
                                               const BlkWork* __restrict__ work, const float4* __restrict__ node4,
                                               const int* __restrict__ child, const int* __restrict__ slotoff,
-                                              int* sorted, int* forder,
-                                              const float4* __restrict__ acc4, int* keys,
-                                              const int* __restrict__ sorder) {
+                                              int* sorted, int* forder, const int* __restrict__ sorder) {
     const BlkWork w = work[blockIdx.x];
     const SysDesc d = sd[w.sys];
     SysState* S = st + w.sys;
@@ -789,12 +771,7 @@ __global__ __launch_bounds__(256) void k_sort(const SysDesc* __restrict__ sd, Sy
         }
         if (v == b) {
             if ((unsigned)rank < (unsigned)n) sorted[d.body_off + rank] = b;
-            if ((unsigned)hrank < (unsigned)n) {
-                forder[d.body_off + hrank] = b;
-                if (keys && (hrank & (kForceVote - 1)) == 0)
-                    atomicMax(keys + d.grp_first + hrank / (kForceVote * kForceWaves),
-                              (int)fminf(fmaxf(acc4[d.body_off + b].w, 0.0f), 1e6f));
-            }
+            if ((unsigned)hrank < (unsigned)n) forder[d.body_off + hrank] = b;
             return;
         }
         if (v < n) break;  // not this body's slot: inconsistent tree
@@ -834,13 +811,17 @@ typedef int v8i __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 
 // sorder (full passes only, else NULL): receives the force order, the order in
-// which the next sort visits the bodies.
+// which the next sort visits the bodies.  keys (full passes with longest-first dispatch,
+// else NULL): per force workgroup, the largest interaction count of its waves, the
+// dispatch-order key of the next pass (a workgroup covers nearly the same bodies from
+// one step to the next).
 template <bool KAHAN, bool PAIR>
 __global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __restrict__ sd, SysState* st,
                                                   const BlkWork* __restrict__ work,
                                                   const float4* __restrict__ node4, const int* __restrict__ child,
                                                   const float4* __restrict__ cdata, const int* __restrict__ sorted,
-                                                  float4* __restrict__ acc4, int* __restrict__ sorder) {
+                                                  float4* __restrict__ acc4, int* __restrict__ sorder,
+                                                  int* __restrict__ keys) {
     constexpr int kS = PAIR ? kStackPair : kStack;
     __shared__ int stk[kForceWaves][kS];
     // the thresholds by level, over every value of a stack entry's 7-bit level field (the
@@ -989,12 +970,13 @@ __global__ __launch_bounds__(64 * kForceWaves) void k_force_bh(const SysDesc* __
         sorder[d.body_off + idx] = b;
         if (idx == 0) st[w.sys].order_ok = 1;
     }
+    if (keys && lane == 0) atomicMax(keys + d.grp_first + w.begin / (kForceVote * kForceWaves), min(nint, 1 << 24));
 }
 
 // Dispatch order of the force workgroups, longest first (LPT list scheduling),
 // for launches of at most two waves per SIMD slot, where the last workgroups to
-// start decide when the kernel ends.  keys[i] (from k_sort; see there) estimates
-// workgroup i's time; workgroups are binned by it (64 bins up to the largest) and
+// start decide when the kernel ends.  keys[i] (written by the previous force pass)
+// estimates workgroup i's time; workgroups are binned by it (64 bins up to the largest) and
 // written out largest bin first, and the keys are cleared for the next pass.  The
 // order decides only when a workgroup starts, never what it computes.  One
 // workgroup of 1024 threads; the keys stay in LDS.
@@ -1034,6 +1016,79 @@ __global__ __launch_bounds__(1024) void k_force_order(const BlkWork* __restrict_
     }
     __syncthreads();
     for (int i = threadIdx.x; i < len; i += blockDim.x) out[atomicAdd(&hist[key[i]], 1)] = in[i];
+}
+
+// The same order computed by one 256-lane workgroup with the keys in registers (at most
+// kOrderSmall workgroups), so that it can run as one workgroup of a step's first launch
+// (k_bounds_clear), beside that launch's other workgroups.
+__device__ __forceinline__ void force_order_small(const BlkWork* __restrict__ in, int len, int* keys,
+                                                  BlkWork* __restrict__ out) {
+    constexpr int K = kOrderSmall / 256;
+    __shared__ int hist[kOrderBins];
+    __shared__ int wmax[4];
+    int kk[K];
+    int m = 0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const int i = (int)threadIdx.x + 256 * j;
+        int k = 0;
+        if (i < len) {
+            k = min(max(keys[i], 0), 1 << 24);
+            keys[i] = 0;
+        }
+        kk[j] = k;
+        m = max(m, k);
+    }
+    m = wave_max_i(m);
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    if (threadIdx.x < kOrderBins) hist[threadIdx.x] = 0;
+    __syncthreads();
+    const long long scale = (long long)max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3])) + 1;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        kk[j] = kOrderBins - 1 - (int)((long long)kk[j] * kOrderBins / scale);
+        if ((int)threadIdx.x + 256 * j < len) atomicAdd(&hist[kk[j]], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int sum = 0;
+        for (int b = 0; b < kOrderBins; ++b) {
+            const int c = hist[b];
+            hist[b] = sum;
+            sum += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const int i = (int)threadIdx.x + 256 * j;
+        if (i < len) out[atomicAdd(&hist[kk[j]], 1)] = in[i];
+    }
+}
+
+// A step's bounds pass and tree clear in one launch: with an order table (olen > 0),
+// workgroup 0 computes the force pass's longest-first dispatch order (force_order_small)
+// beside the others; of the rest, those below nb take the bounds table (their ticket counts
+// only them) and the others the cell table.  The root's re-seeding (k_tree_init's work) is
+// then done by k_build_top, the build's first launch.
+__global__ __launch_bounds__(256) void k_bounds_clear(const SysDesc* __restrict__ sd, SysState* st,
+                                                      const BlkWork* __restrict__ wb, int nb,
+                                                      const BlkWork* __restrict__ wc,
+                                                      const float4* __restrict__ node4, float* partial, int* child,
+                                                      int* cpend, const BlkWork* __restrict__ og, int olen, int* keys,
+                                                      BlkWork* __restrict__ oout) {
+    int blk = blockIdx.x;
+    if (olen > 0) {
+        if (blk == 0) {
+            force_order_small(og, olen, keys, oout);
+            return;
+        }
+        --blk;
+    }
+    if (blk < nb)
+        bounds_block(sd, st, wb[blk], blk, node4, partial);
+    else
+        tree_clear_block(sd, st, wc[blk - nb], child, cpend);
 }
 
 // Block table of one target range [lo, hi) of system `sys`: blocks of `per` items
@@ -1438,8 +1493,10 @@ void launch_tree_clear(hipStream_t s, int nblk, const SysDesc* sd, const SysStat
 }
 void launch_bounds_clear(hipStream_t s, int nbounds, const BlkWork* wb, int ncells, const BlkWork* wc,
                          const SysDesc* sd, SysState* st, const float4* node4, float* partial, int* child,
-                         int* cpend) {
-    NB_LAUNCH(k_bounds_clear, nbounds + ncells, 256, s, sd, st, wb, nbounds, wc, node4, partial, child, cpend);
+                         int* cpend, const BlkWork* og, int olen, int* keys, BlkWork* oout) {
+    if (olen > kOrderSmall) olen = 0;  // the caller orders with k_force_order then
+    NB_LAUNCH(k_bounds_clear, (olen > 0 ? 1 : 0) + nbounds + ncells, 256, s, sd, st, wb, nbounds, wc, node4, partial,
+              child, cpend, og, olen, keys, oout);
 }
 void launch_tree_init(hipStream_t s, int nsys, const int* slots, const SysDesc* sd, SysState* st, int* slotoff) {
     if (nsys <= 0) return;
@@ -1471,19 +1528,19 @@ void launch_summarize(hipStream_t s, int nblk, const SysDesc* sd, const SysState
     NB_LAUNCH(k_summarize, nblk, 256, s, sd, st, work, node4, child, cparent, cpend, ccount, slotoff, cdata);
 }
 void launch_sort(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work, const float4* node4,
-                 const int* child, const int* slotoff, int* sorted, int* forder, const float4* acc4, int* keys,
-                 const int* sorder) {
-    NB_LAUNCH(k_sort, nblk, 256, s, sd, st, work, node4, child, slotoff, sorted, forder, acc4, keys, sorder);
+                 const int* child, const int* slotoff, int* sorted, int* forder, const int* sorder) {
+    NB_LAUNCH(k_sort, nblk, 256, s, sd, st, work, node4, child, slotoff, sorted, forder, sorder);
 }
 void launch_force_order(hipStream_t s, const BlkWork* in, int len, int* keys, BlkWork* out) {
     if (len > 0 && len <= kOrderMax) NB_LAUNCH(k_force_order, 1, 1024, s, in, len, keys, out);
 }
 void launch_force_bh(hipStream_t s, int nblk, bool kahan, bool pair, const SysDesc* sd, SysState* st,
                      const BlkWork* work, const float4* node4, const int* child, const float4* cdata,
-                     const int* order, float4* acc4, int* sorder) {
+                     const int* order, float4* acc4, int* sorder, int* keys) {
 #define NB_FBH(K, P)                                                                                          \
     if (kahan == K && pair == P)                                                                              \
-    NB_LAUNCH((k_force_bh<K, P>), nblk, 64 * kForceWaves, s, sd, st, work, node4, child, cdata, order, acc4, sorder)
+    NB_LAUNCH((k_force_bh<K, P>), nblk, 64 * kForceWaves, s, sd, st, work, node4, child, cdata, order, acc4, sorder, \
+              keys)
     NB_FBH(false, false); NB_FBH(true, false); NB_FBH(false, true); NB_FBH(true, true);
 #undef NB_FBH
 }
