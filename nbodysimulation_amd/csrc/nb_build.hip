@@ -92,6 +92,7 @@ __global__ __launch_bounds__(256) void k_classify(const SysDesc* __restrict__ sd
         S->nframes = 0;
         S->listpos = 0;
         S->bq = 0;
+        if (stride == 1) S->nfb1 = S->nfb;  // the last phase: bodies earlier phases left
     }
     const int n = d.n;
     const int i = w.begin + (int)threadIdx.x;  // the phase's enumeration: bkey/brank are indexed by i
@@ -299,10 +300,11 @@ __global__ __launch_bounds__(256) void k_scatter(const SysDesc* __restrict__ sd,
 struct FrameLds {
     int c[8 * kFrameCells];  // slots: -1 empty, [0, kFrame) an entry of the frame, >= kCellTag a cell
     int par[kFrameCells];    // (parent cell << 3) | slot; a bucket's root: -1 - key (its slot of T)
-    int cont[kFrameCells];   // bit k: slot k claimed by more than one body this level
+    int cont[kFrameCells];   // bit k: slot k claimed by more than one body this level; bits 8..: the cell's level
     int body[kFrame];        // entry -> body
     int key[kFrame];         // entry -> bucket key; for a leader, later its bucket's root cell
     int wmax[kFrameThreads / 64];
+    int wmin[kFrameThreads / 64];
     int nc, fail, base;
 };
 
@@ -311,8 +313,8 @@ struct FrameLds {
 // (four waves per SIMD): left free the compiler takes 129, which allows three, fewer
 // workgroups per CU than the LDS allows (eight).
 __global__ __launch_bounds__(kFrameThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_frame_build(
-    const SysDesc* __restrict__ sd, SysState* st, const BlkWork* __restrict__ work, const float4* __restrict__ node4,
-    int* child, int* cparent, int* cpend, int* slotoff, float4* cdata, const float4* __restrict__ lpos,
+    const SysDesc* __restrict__ sd, SysState* st, const BlkWork* __restrict__ work, float4* node4, int* child,
+    int* cparent, int* cpend, int* ccount, int* slotoff, float4* cdata, const float4* __restrict__ lpos,
     const int4* __restrict__ lmeta, const int2* __restrict__ bframe, int* fblist, int last) {
     __shared__ FrameLds L;
     const BlkWork w = work[blockIdx.x];
@@ -327,6 +329,9 @@ __global__ __launch_bounds__(kFrameThreads) __attribute__((amdgpu_waves_per_eu(4
     float rlx, rly, rlz, rhx, rhy, rhz;
     root_box(d.rules, S->box, rlx, rly, rlz, rhx, rhy, rhz);
     int depth = 1;
+    // The last phase's frames also summarize their subtrees, unless earlier phases left bodies
+    // for the queued insertion (which could then land in them): nfb1 counts those.
+    const bool summ = last && ld_c(&S->nfb1) == 0;
     for (int f = w.begin; f < nf; f += w.end) {  // uniform over the block
         const int2 fr = bframe[d.body_off + f];
         const int size = min(max(fr.y, 0), kFrame);
@@ -398,7 +403,7 @@ __global__ __launch_bounds__(kFrameThreads) __attribute__((amdgpu_waves_per_eu(4
                 const int c = min(atomicAdd(&L.nc, 1), kFrameCells - 1);  // at most kFrame / 2 roots
                 for (int k = 0; k < 8; ++k) L.c[8 * c + k] = kEmpty;
                 L.par[c] = -1 - ek[t];
-                L.cont[c] = 0;
+                L.cont[c] = (lv0[t] + 1) << 8;
                 L.key[4 * tid + t] = c;  // read only by this bucket's entries from here on
             }
         }
@@ -441,7 +446,7 @@ __global__ __launch_bounds__(kFrameThreads) __attribute__((amdgpu_waves_per_eu(4
                     } else {
                         for (int k = 0; k < 8; ++k) L.c[8 * c + k] = kEmpty;
                         L.par[c] = (node[t] << 3) | dgs[t];
-                        L.cont[c] = 0;
+                        L.cont[c] = (lvl[t] + 1) << 8;
                         L.c[slot[t]] = kCellTag + c;
                     }
                 }
@@ -487,8 +492,8 @@ __global__ __launch_bounds__(kFrameThreads) __attribute__((amdgpu_waves_per_eu(4
             }
             reinterpret_cast<int4*>(C + 8 * (size_t)g)[0] = make_int4(v[0], v[1], v[2], v[3]);
             reinterpret_cast<int4*>(C + 8 * (size_t)g)[1] = make_int4(v[4], v[5], v[6], v[7]);
-            cpend[d.cell_off + g] = np;
             const int pv = L.par[c];
+            cpend[d.cell_off + g] = summ ? (pv < 0 ? kSumRoot : kSumDone) : np;
             cparent[d.cell_off + g] = pv < 0 ? (-1 - pv) : (((base + (pv >> 3)) << 3) | (pv & 7));
             if (!last) {  // a later phase counts its buckets here
                 int4* so = reinterpret_cast<int4*>(slotoff + 8 * ((size_t)d.cell_off + g));
@@ -509,6 +514,80 @@ __global__ __launch_bounds__(kFrameThreads) __attribute__((amdgpu_waves_per_eu(4
                 C[ek[t]] = eb[t];
                 CD[ek[t]] = p[t];
                 depth = max(depth, lv0[t]);
+            }
+        }
+        if (summ) {
+            // The subtrees summarized bottom-up, one level per round, each cell's children in
+            // slot order with k_summarize's arithmetic (so the same bits): a cell's (COM, mass)
+            // and body count go to its parent's child block and per-slot counts (for a bucket's
+            // root, its slot of T, where k_summarize reads them) and into the cell's own first
+            // LDS slots, read by nothing else once the cell is done, for its parent's round.
+            __syncthreads();  // every cell's slots are written out (above)
+            int lmax = 0, lmin = kMaxLevel + 1;
+            for (int c = tid; c < nc; c += kFrameThreads) {
+                const int lv = L.cont[c] >> 8;
+                lmax = max(lmax, lv);
+                lmin = min(lmin, lv);
+            }
+            lmax = wave_max_i(lmax);
+            lmin = -wave_max_i(-lmin);
+            if (lane == 0) {
+                L.wmax[wave] = lmax;
+                L.wmin[wave] = lmin;
+            }
+            __syncthreads();
+            lmax = max(L.wmax[0], L.wmax[1]);
+            lmin = min(L.wmin[0], L.wmin[1]);
+            const float4* FP = lpos + d.body_off + fr.x;
+            int* SO = slotoff + 8 * (size_t)d.cell_off;
+            for (int lv = lmax; lv >= lmin; --lv) {  // uniform
+                for (int c = tid; c < nc; c += kFrameThreads) {
+                    if ((L.cont[c] >> 8) != lv) continue;
+                    int sv[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) sv[k] = L.c[8 * c + k];
+                    float m = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+                    int cnt = 0, incl[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        const int v = sv[k];
+                        if (v != kEmpty) {
+                            float4 q;
+                            int cc = 1;
+                            if (v >= kCellTag) {  // a child cell, summarized in an earlier round
+                                const int* s2 = L.c + 8 * min(v - kCellTag, kFrameCells - 1);
+                                q = make_float4(__int_as_float(s2[0]), __int_as_float(s2[1]), __int_as_float(s2[2]),
+                                                __int_as_float(s2[3]));
+                                cc = s2[4];
+                            } else {
+                                q = FP[min(v, size - 1)];  // a valid entry of this frame
+                            }
+                            const float wm = q.w;
+                            cnt += cc;
+                            m = m + wm;
+                            cx = fmaf(q.x, wm, cx);
+                            cy = fmaf(q.y, wm, cy);
+                            cz = fmaf(q.z, wm, cz);
+                        }
+                        incl[k] = cnt;
+                    }
+                    const float inv = 1.0f / m;
+                    const float4 com = make_float4(cx * inv, cy * inv, cz * inv, m);
+                    const int g = base + c;
+                    int4* so = reinterpret_cast<int4*>(SO + 8 * (size_t)g);
+                    so[0] = make_int4(incl[0], incl[1], incl[2], incl[3]);
+                    so[1] = make_int4(incl[4], incl[5], incl[6], incl[7]);
+                    const int pv = L.par[c];
+                    const size_t e = pv >= 0 ? 8 * (size_t)(base + (pv >> 3)) + (pv & 7) : (size_t)(-1 - pv);
+                    CD[e] = com;
+                    SO[e] = cnt;
+                    L.c[8 * c] = __float_as_int(com.x);
+                    L.c[8 * c + 1] = __float_as_int(com.y);
+                    L.c[8 * c + 2] = __float_as_int(com.z);
+                    L.c[8 * c + 3] = __float_as_int(com.w);
+                    L.c[8 * c + 4] = cnt;
+                }
+                __syncthreads();
             }
         }
         __syncthreads();  // the next frame reuses the LDS
@@ -539,11 +618,12 @@ void launch_scatter(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, co
                            sorder, lpos, lmeta, fblist, stride, exclude);
 }
 void launch_frame_build(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
-                        const float4* node4, int* child, int* cparent, int* cpend, int* slotoff, float4* cdata,
-                        const float4* lpos, const int4* lmeta, const int2* bframe, int* fblist, bool last) {
+                        float4* node4, int* child, int* cparent, int* cpend, int* ccount, int* slotoff,
+                        float4* cdata, const float4* lpos, const int4* lmeta, const int2* bframe, int* fblist,
+                        bool last) {
     if (nblk > 0)
         hipLaunchKernelGGL(k_frame_build, dim3(nblk), dim3(kFrameThreads), 0, s, sd, st, work, node4, child,
-                           cparent, cpend, slotoff, cdata, lpos, lmeta, bframe, fblist, last ? 1 : 0);
+                           cparent, cpend, ccount, slotoff, cdata, lpos, lmeta, bframe, fblist, last ? 1 : 0);
 }
 
 }  // namespace nb

@@ -30,6 +30,10 @@ void launch_tree_clear(hipStream_t s, int nblk, const SysDesc* sd, const SysStat
 void launch_bounds_clear(hipStream_t s, int nbounds, const BlkWork* wb, int ncells, const BlkWork* wc,
                          const SysDesc* sd, SysState* st, const float4* node4, float* partial, int* child,
                          int* cpend, const BlkWork* og, int olen, int* keys, BlkWork* oout);
+// one system's per-cell (COM, mass) (cells: its node4 past the bodies) and body counts, for the
+// exports (the step writes them for the root only)
+void launch_cell_export(hipStream_t s, int n, int ncells, const int* cparent, const float4* cdata,
+                        const int* slotoff, float4* cells, int* ccount);
 // slots NULL: 0..nsys-1; sd and slotoff NULL: the roots' per-slot counts are left alone
 void launch_tree_init(hipStream_t s, int nsys, const int* slots, const SysDesc* sd, SysState* st, int* slotoff);
 // fblist: the build's fallback list, which takes the phase's bodies when the LDS tree overflows
@@ -51,9 +55,12 @@ void launch_scatter(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, co
                     const float4* ppos, const int* kbase, const int* bkey, const int* brank, const int* ccount,
                     const int* sorder, float4* lpos, int4* lmeta, int* fblist, int stride, int exclude);
 // last: no later bucketed phase (the new cells' per-slot counts are not cleared)
+// last: the last phase, whose frames also summarize their subtrees (when no earlier phase
+// left bodies for the queued insertion, which could land in them)
 void launch_frame_build(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
-                        const float4* node4, int* child, int* cparent, int* cpend, int* slotoff, float4* cdata,
-                        const float4* lpos, const int4* lmeta, const int2* bframe, int* fblist, bool last);
+                        float4* node4, int* child, int* cparent, int* cpend, int* ccount, int* slotoff,
+                        float4* cdata, const float4* lpos, const int4* lmeta, const int2* bframe, int* fblist,
+                        bool last);
 void launch_summarize(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
                       float4* node4, const int* child, const int* cparent, int* cpend, int* ccount, int* slotoff,
                       float4* cdata);

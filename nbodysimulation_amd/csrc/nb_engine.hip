@@ -408,7 +408,7 @@ void Engine::build() {
         launch_scatter(stream, tab_len[bt], d_desc, d_state, tab[bt], ppos, kbase, sorted, forder, ccount, sorder,
                        lpos, lmeta, fblist, stride, exclude);
         launch_frame_build(stream, tab_len[T_FRAMES], d_desc, d_state, tab[T_FRAMES], node4, child, cparent, cpend,
-                           slotoff, cdata, lpos, lmeta, bframe, fblist, ph == kBuildPhases - 1);
+                           ccount, slotoff, cdata, lpos, lmeta, bframe, fblist, ph == kBuildPhases - 1);
     }
     // the bodies every phase left (the fallback list: a frame over the LDS pool or the depth
     // limit, a bucket over the frame budget, a first phase over the LDS cell pool), inserted

@@ -55,6 +55,12 @@ constexpr int kEnergyLen = 9;   // kinetic, potential, p[3], L[3], mass
 constexpr int kEmpty = -1;
 constexpr int kLocked = -2;
 
+// cpend markers of cells that the last phase's frame build summarized (their (COM, mass),
+// counts and child blocks are final): summarize skips kSumDone; from kSumRoot (a bucket's
+// root) it only hands the count down to the cell above
+constexpr int kSumDone = -(1 << 30);
+constexpr int kSumRoot = -(1 << 30) + 1;
+
 // status bits (SysState::status)
 constexpr int kStTreeOverflow = 1;
 constexpr int kStDepthOverflow = 2;
@@ -96,7 +102,7 @@ struct SysState {
     int order_ok;   // 1: sorder holds a permutation of the bodies (the last full force pass's order)
     int top;        // this step's first insertion phase: 1 built in LDS (k_build_top); 2 its bodies
                     // went to the fallback list (the LDS cell pool overflowed); 0 neither
-    int pad1;
+    int nfb1;       // the fallback list's length when the last bucketed phase began
     // a bucketed build phase: frames and frame-list entries claimed (the two halves of one
     // 8-byte-aligned word, claimed together), and bodies sent to the fallback insertion
     int nframes;

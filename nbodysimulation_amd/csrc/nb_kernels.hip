@@ -610,7 +610,12 @@ __global__ __launch_bounds__(256) void k_summarize(const SysDesc* __restrict__ s
     if (ci >= w.end || ci >= used) return;
     const int n = d.n;
     const int* C = child + 8 * (size_t)d.cell_off;
-    {
+    // a cell of a subtree the frame build summarized: done; that subtree's root only hands its
+    // count down to the cell above (its (COM, mass) and count are already in that cell's blocks)
+    const int pe = cpend[d.cell_off + ci];
+    if (pe == kSumDone) return;
+    const bool sroot = pe == kSumRoot;
+    if (!sroot) {
         const int4* c4 = reinterpret_cast<const int4*>(C + 8 * ci);
         int4 a = c4[0], bq = c4[1];
         int mx = max(max(max(a.x, a.y), max(a.z, a.w)), max(max(bq.x, bq.y), max(bq.z, bq.w)));
@@ -620,6 +625,13 @@ __global__ __launch_bounds__(256) void k_summarize(const SysDesc* __restrict__ s
     int* SO = slotoff + 8 * (size_t)d.cell_off;
     int c = ci;
     for (bool first = true;; first = false) {
+        if (first && sroot) {
+            const int p = cparent[d.cell_off + c] >> 3;
+            if (__hip_atomic_fetch_add(cpend + d.cell_off + p, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 1)
+                break;
+            c = p;
+            continue;
+        }
         // The cell's slots and parent link are final since the build (ordinary cached loads).
         // A lane's first cell has only bodies below it, whose child-block entries the build
         // wrote: cached loads, and no counts (a body counts 1).  A cell it goes on with also
@@ -709,8 +721,10 @@ __global__ __launch_bounds__(256) void k_summarize(const SysDesc* __restrict__ s
         // the cell's own outputs: read by later kernels only
         so[0] = make_int4(incl[0], incl[1], incl[2], incl[3]);
         so[1] = make_int4(incl[4], incl[5], incl[6], incl[7]);
-        node4[d.node_off + n + c] = com;
-        ccount[d.cell_off + c] = cnt;
+        if (c == 0) {  // another cell's (COM, mass) and count are in its parent's blocks (k_cell_export)
+            node4[d.node_off + n] = com;
+            ccount[d.cell_off] = cnt;
+        }
         if (!more) break;
         c = p;
     }
@@ -1089,6 +1103,19 @@ __global__ __launch_bounds__(256) void k_bounds_clear(const SysDesc* __restrict_
         bounds_block(sd, st, wb[blk], blk, node4, partial);
     else
         tree_clear_block(sd, st, wc[blk - nb], child, cpend);
+}
+
+// The per-cell (COM, mass) and body count of one system's tree for the exports, from the
+// blocks the step writes: a cell's (COM, mass) is its entry in its parent's child block and its
+// count its last inclusive per-slot count (the root's were written by k_summarize).
+__global__ __launch_bounds__(256) void k_cell_export(int n, int ncells, const int* __restrict__ cparent,
+                                                     const float4* __restrict__ cdata,
+                                                     const int* __restrict__ slotoff, float4* cells, int* ccount) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < 1 || c >= ncells) return;
+    const int pk = cparent[c];
+    cells[c] = cdata[8 * (size_t)(pk >> 3) + (pk & 7)];
+    ccount[c] = slotoff[8 * (size_t)c + 7];
 }
 
 // Block table of one target range [lo, hi) of system `sys`: blocks of `per` items
@@ -1497,6 +1524,10 @@ void launch_bounds_clear(hipStream_t s, int nbounds, const BlkWork* wb, int ncel
     if (olen > kOrderSmall) olen = 0;  // the caller orders with k_force_order then
     NB_LAUNCH(k_bounds_clear, (olen > 0 ? 1 : 0) + nbounds + ncells, 256, s, sd, st, wb, nbounds, wc, node4, partial,
               child, cpend, og, olen, keys, oout);
+}
+void launch_cell_export(hipStream_t s, int n, int ncells, const int* cparent, const float4* cdata,
+                        const int* slotoff, float4* cells, int* ccount) {
+    NB_LAUNCH(k_cell_export, (ncells + 255) / 256, 256, s, n, ncells, cparent, cdata, slotoff, cells, ccount);
 }
 void launch_tree_init(hipStream_t s, int nsys, const int* slots, const SysDesc* sd, SysState* st, int* slotoff) {
     if (nsys <= 0) return;
