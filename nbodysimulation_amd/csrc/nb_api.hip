@@ -841,7 +841,7 @@ static nb_status fetch_tree(nb_sim* sim, HostTree& t) {
     t.count.resize(t.ncells);
     t.cells.resize(t.ncells);
     const nb::SysDesc& d = e.hdesc[0];
-    nb::launch_cell_export(e.stream, d.n, t.ncells, e.cparent + d.cell_off, e.cdata + 8 * (size_t)d.cell_off,
+    nb::launch_cell_export(e.stream, d.n, t.ncells, e.child + 8 * (size_t)d.cell_off, e.cdata + 8 * (size_t)d.cell_off,
                            e.slotoff + 8 * (size_t)d.cell_off, e.node4 + d.node_off + d.n, e.ccount + d.cell_off);
     NB_HIPC(hipGetLastError());
     NB_HIPC(hipMemcpyAsync(t.child.data(), e.child + 8 * (size_t)d.cell_off, sizeof(int) * 8 * t.ncells,

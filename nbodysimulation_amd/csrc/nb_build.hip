@@ -494,7 +494,8 @@ __global__ __launch_bounds__(kFrameThreads) __attribute__((amdgpu_waves_per_eu(4
             reinterpret_cast<int4*>(C + 8 * (size_t)g)[1] = make_int4(v[4], v[5], v[6], v[7]);
             const int pv = L.par[c];
             cpend[d.cell_off + g] = summ ? (pv < 0 ? kSumRoot : kSumDone) : np;
-            cparent[d.cell_off + g] = pv < 0 ? (-1 - pv) : (((base + (pv >> 3)) << 3) | (pv & 7));
+            // parent link (k_summarize's; a summarized cell's only for a bucket root's count)
+            if (!summ || pv < 0) cparent[d.cell_off + g] = pv < 0 ? (-1 - pv) : (((base + (pv >> 3)) << 3) | (pv & 7));
             if (!last) {  // a later phase counts its buckets here
                 int4* so = reinterpret_cast<int4*>(slotoff + 8 * ((size_t)d.cell_off + g));
                 so[0] = make_int4(0, 0, 0, 0);

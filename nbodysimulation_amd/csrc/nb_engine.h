@@ -32,7 +32,7 @@ void launch_bounds_clear(hipStream_t s, int nbounds, const BlkWork* wb, int ncel
                          int* cpend, const BlkWork* og, int olen, int* keys, BlkWork* oout);
 // one system's per-cell (COM, mass) (cells: its node4 past the bodies) and body counts, for the
 // exports (the step writes them for the root only)
-void launch_cell_export(hipStream_t s, int n, int ncells, const int* cparent, const float4* cdata,
+void launch_cell_export(hipStream_t s, int n, int ncells, const int* child, const float4* cdata,
                         const int* slotoff, float4* cells, int* ccount);
 // slots NULL: 0..nsys-1; sd and slotoff NULL: the roots' per-slot counts are left alone
 void launch_tree_init(hipStream_t s, int nsys, const int* slots, const SysDesc* sd, SysState* st, int* slotoff);

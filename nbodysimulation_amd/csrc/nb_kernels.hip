@@ -1107,15 +1107,21 @@ __global__ __launch_bounds__(256) void k_bounds_clear(const SysDesc* __restrict_
 
 // The per-cell (COM, mass) and body count of one system's tree for the exports, from the
 // blocks the step writes: a cell's (COM, mass) is its entry in its parent's child block and its
-// count its last inclusive per-slot count (the root's were written by k_summarize).
-__global__ __launch_bounds__(256) void k_cell_export(int n, int ncells, const int* __restrict__ cparent,
+// count its last inclusive per-slot count (the root's were written by k_summarize).  One lane
+// per cell hands its child cells their entries (no parent links needed).
+__global__ __launch_bounds__(256) void k_cell_export(int n, int ncells, const int* __restrict__ child,
                                                      const float4* __restrict__ cdata,
                                                      const int* __restrict__ slotoff, float4* cells, int* ccount) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < 1 || c >= ncells) return;
-    const int pk = cparent[c];
-    cells[c] = cdata[8 * (size_t)(pk >> 3) + (pk & 7)];
-    ccount[c] = slotoff[8 * (size_t)c + 7];
+    if (c >= ncells) return;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int v = child[8 * (size_t)c + k] - n;
+        if (v > 0 && v < ncells) {
+            cells[v] = cdata[8 * (size_t)c + k];
+            ccount[v] = slotoff[8 * (size_t)v + 7];
+        }
+    }
 }
 
 // Block table of one target range [lo, hi) of system `sys`: blocks of `per` items
@@ -1525,9 +1531,9 @@ void launch_bounds_clear(hipStream_t s, int nbounds, const BlkWork* wb, int ncel
     NB_LAUNCH(k_bounds_clear, (olen > 0 ? 1 : 0) + nbounds + ncells, 256, s, sd, st, wb, nbounds, wc, node4, partial,
               child, cpend, og, olen, keys, oout);
 }
-void launch_cell_export(hipStream_t s, int n, int ncells, const int* cparent, const float4* cdata,
+void launch_cell_export(hipStream_t s, int n, int ncells, const int* child, const float4* cdata,
                         const int* slotoff, float4* cells, int* ccount) {
-    NB_LAUNCH(k_cell_export, (ncells + 255) / 256, 256, s, n, ncells, cparent, cdata, slotoff, cells, ccount);
+    NB_LAUNCH(k_cell_export, (ncells + 255) / 256, 256, s, n, ncells, child, cdata, slotoff, cells, ccount);
 }
 void launch_tree_init(hipStream_t s, int nsys, const int* slots, const SysDesc* sd, SysState* st, int* slotoff) {
     if (nsys <= 0) return;
