@@ -579,9 +579,11 @@ __global__ __launch_bounds__(kFrameThreads) __attribute__((amdgpu_waves_per_eu(4
                     so[0] = make_int4(incl[0], incl[1], incl[2], incl[3]);
                     so[1] = make_int4(incl[4], incl[5], incl[6], incl[7]);
                     const int pv = L.par[c];
+                    // the parent's child block entry; the count only for a bucket's root (a frame
+                    // cell's per-slot counts are its own inclusive ones, written above)
                     const size_t e = pv >= 0 ? 8 * (size_t)(base + (pv >> 3)) + (pv & 7) : (size_t)(-1 - pv);
                     CD[e] = com;
-                    SO[e] = cnt;
+                    if (pv < 0) SO[e] = cnt;
                     L.c[8 * c] = __float_as_int(com.x);
                     L.c[8 * c + 1] = __float_as_int(com.y);
                     L.c[8 * c + 2] = __float_as_int(com.z);
