@@ -152,15 +152,18 @@ __global__ __launch_bounds__(256) void k_classify(const SysDesc* __restrict__ sd
 
 // One lane per slot of T (grid-stride: the block is w.begin of the system's w.end blocks and
 // takes 256 consecutive slots at a time).  Each wave packs its 64 slots' buckets greedily in
-// slot order (one lane), then the block claims its waves' frames and list space with one
-// 64-bit atomic add (frames low, list entries high): same-address atomics cost about as
+// slot order: a frame takes buckets while its total stays within the budget.  With the
+// buckets' sizes prefix-summed over the wave, the frame that starts at list offset `base`
+// holds exactly the buckets whose inclusive sum exceeds `base` by at most the budget, so a
+// frame costs one ballot, not one step per bucket, and a bucket's place in the wave's list
+// region is its exclusive sum.  The block then claims its waves' frames and list space with
+// one 64-bit atomic add (frames low, list entries high): same-address atomics cost about as
 // much as a wave's whole packing, so there is one per block and round.
 __global__ __launch_bounds__(256) void k_frames(const SysDesc* __restrict__ sd, SysState* st,
                                                 const BlkWork* __restrict__ work, const float4* __restrict__ node4,
                                                 const int* __restrict__ child, int* slotoff,
                                                 const int* __restrict__ ccount, int* kbase, float4* lpos, int4* lmeta,
                                                 int2* bframe) {
-    __shared__ int s_sz[4][64], s_f[4][64], s_o[4][64], s_fo[4][64], s_fs[4][64];
     __shared__ int s_nf[4], s_tot[4], s_fb, s_lb, s_ok;
     const BlkWork w = work[blockIdx.x];
     const SysDesc d = sd[w.sys];
@@ -178,59 +181,61 @@ __global__ __launch_bounds__(256) void k_frames(const SysDesc* __restrict__ sd, 
     int2* frames = bframe + d.body_off;
     for (int k0 = w.begin * 256; k0 < nkeys; k0 += w.end * 256) {  // uniform over the block
         const int k = k0 + (int)threadIdx.x;
-        int cnt = 0, res = -1;
+        // the slot's count, its content and its cell's level, loaded together
+        int cnt = 0, v = -1, lvl = 0;
         if (k < nkeys) {
             cnt = cnt8[k];
-            if (cnt) {
-                cnt8[k] = 0;  // cleared for the next phase's counts
-                const int v = C[k];
-                res = (v >= 0 && v < n) ? v : -1;  // the body already in the slot joins the bucket
-            }
+            v = C[k];
+            lvl = ccount[d.cell_off + (k >> 3)];
         }
+        const int res = (cnt && v >= 0 && v < n) ? v : -1;  // the body already in the slot joins the bucket
+        float4 rp = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (res >= 0) rp = node4[d.node_off + res];
+        if (cnt) cnt8[k] = 0;  // cleared for the next phase's counts
         const int size = cnt ? cnt + (res >= 0 ? 1 : 0) : 0;
         const bool big = size > budget;
         const int sz = big ? 0 : size;
-        s_sz[wave][lane] = sz;
-        wave_sync();
-        if (lane == 0) {  // greedy packing of the wave's slots in slot order
-            int f = 0, used = 0, tot = 0;
-            for (int j = 0; j < 64; ++j) {
-                const int sj = s_sz[wave][j];
-                if (sj == 0) continue;
-                if (used + sj > budget) {
-                    s_fs[wave][f++] = used;
-                    used = 0;
-                }
-                s_f[wave][j] = f;
-                s_o[wave][j] = used;
-                used += sj;
-                tot += sj;
+        int incl = sz;  // inclusive sum over the wave's lanes
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(incl, o);
+            if (lane >= o) incl += y;
+        }
+        const int excl = incl - sz;
+        const int tot = __shfl(incl, 63);
+        // frames of the wave: lane q < nf receives frame q's offset and size
+        int nf = 0, fbeg = 0, fsz = 0;
+        for (int base = 0; base < tot; ++nf) {  // uniform: base, tot and nf are the same in every lane
+            const bool in = sz > 0 && excl >= base && incl - base <= budget;
+            const unsigned long long m = __ballot(in);
+            const int last = 63 - __clzll((long long)m);  // in-lanes are contiguous (incl rises)
+            const int end = __shfl(incl, last);
+            if (lane == nf) {
+                fbeg = base;
+                fsz = end - base;
             }
-            int nf = f;
-            if (used > 0) s_fs[wave][nf++] = used;
-            for (int q = 0, o = 0; q < nf; ++q) {
-                s_fo[wave][q] = o;
-                o += s_fs[wave][q];
-            }
+            base = end;
+        }
+        if (lane == 0) {
             s_nf[wave] = nf;
             s_tot[wave] = tot;
         }
         __syncthreads();
         if (threadIdx.x == 0) {  // one claim for the block: frames low, list entries high
-            int nf = 0, tot = 0;
+            int bnf = 0, btot = 0;
             for (int q = 0; q < 4; ++q) {
-                nf += s_nf[q];
-                tot += s_tot[q];
+                bnf += s_nf[q];
+                btot += s_tot[q];
             }
             unsigned long long old = 0;
-            if (nf > 0)
+            if (bnf > 0)
                 old = atomicAdd(reinterpret_cast<unsigned long long*>(&S->nframes),
-                                (unsigned long long)(unsigned)nf | ((unsigned long long)(unsigned)tot << 32));
+                                (unsigned long long)(unsigned)bnf | ((unsigned long long)(unsigned)btot << 32));
             s_fb = (int)(unsigned)old;
             s_lb = (int)(unsigned)(old >> 32);
             // frames <= buckets <= bodies of the phase; the list holds the phase's bodies and the
             // bodies of earlier phases already in the buckets' slots, so at most n entries
-            s_ok = !(nf > 0 && (s_fb + nf > n || s_lb + tot > n));
+            s_ok = !(bnf > 0 && (s_fb + bnf > n || s_lb + btot > n));
             if (!s_ok) atomicOr(&S->status, kStStall);
         }
         __syncthreads();
@@ -240,19 +245,18 @@ __global__ __launch_bounds__(256) void k_frames(const SysDesc* __restrict__ sd, 
             fb += s_nf[q];
             lb += s_tot[q];
         }
-        const int nf = s_nf[wave];
         if (sz > 0) {
-            const int pos = lb + s_fo[wave][s_f[wave][lane]] + s_o[wave][lane];
+            const int pos = lb + excl;
             if (res >= 0) {
-                LP[pos] = node4[d.node_off + res];
-                LM[pos] = make_int4(res, k, ccount[d.cell_off + (k >> 3)], 0);
+                LP[pos] = rp;
+                LM[pos] = make_int4(res, k, lvl, 0);
             }
             kb[k] = pos + (res >= 0 ? 1 : 0);
         } else if (big) {
             kb[k] = -1;  // its bodies go to the fallback insertion; the body in the slot stays
         }
-        if (lane < nf) frames[fb + lane] = make_int2(lb + s_fo[wave][lane], s_fs[wave][lane]);
-        __syncthreads();  // the next round reuses the LDS rows
+        if (lane < nf) frames[fb + lane] = make_int2(lb + fbeg, fsz);
+        __syncthreads();  // the next round rewrites s_nf and s_tot
     }
 }
 
