@@ -307,19 +307,23 @@ struct FrameLds {
     int cont[kFrameCells];   // bit k: slot k claimed by more than one body this level; bits 8..: the cell's level
     int body[kFrame];        // entry -> body
     int key[kFrame];         // entry -> bucket key; for a leader, later its bucket's root cell
-    int wmax[kFrameThreads / 64];
-    int wmin[kFrameThreads / 64];
+    int wmax[kFrame / 64];  // per wave (up to one entry per lane)
+    int wmin[kFrame / 64];
     int nc, fail, base;
 };
 
-// One workgroup per frame (grid-stride over the system's frames): entries e = 4 * thread + t
+// One workgroup per frame (grid-stride over the system's frames): entries e = EPT * thread + t
 // in frame order, every bucket's subtree built at once, level by level.  At most 128 VGPRs
 // (four waves per SIMD): left free the compiler takes 129, which allows three, fewer
-// workgroups per CU than the LDS allows (eight).
-__global__ __launch_bounds__(kFrameThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_frame_build(
+// workgroups per CU than the LDS allows (eight).  The last phase (thousands of frames, each
+// CU full) runs four entries per lane; the strided phase (a few dozen frames, one per CU)
+// runs one, which shortens each level's chain of dependent steps.
+template <int EPT>  // entries per lane
+__global__ __launch_bounds__(kFrame / EPT) __attribute__((amdgpu_waves_per_eu(4))) void k_frame_build(
     const SysDesc* __restrict__ sd, SysState* st, const BlkWork* __restrict__ work, float4* node4, int* child,
     int* cparent, int* cpend, int* ccount, int* slotoff, float4* cdata, const float4* __restrict__ lpos,
     const int4* __restrict__ lmeta, const int2* __restrict__ bframe, int* fblist, int last) {
+    constexpr int kThreads = kFrame / EPT;
     __shared__ FrameLds L;
     const BlkWork w = work[blockIdx.x];
     const SysDesc d = sd[w.sys];
@@ -339,13 +343,13 @@ __global__ __launch_bounds__(kFrameThreads) __attribute__((amdgpu_waves_per_eu(4
     for (int f = w.begin; f < nf; f += w.end) {  // uniform over the block
         const int2 fr = bframe[d.body_off + f];
         const int size = min(max(fr.y, 0), kFrame);
-        int eb[4], ek[4], node[4], lvl[4], slot[4], dgs[4], lv0[4];
-        float4 p[4];
-        float bx[4][6];
-        bool act[4], valid[4];
+        int eb[EPT], ek[EPT], node[EPT], lvl[EPT], slot[EPT], dgs[EPT], lv0[EPT];
+        float4 p[EPT];
+        float bx[EPT][6];
+        bool act[EPT], valid[EPT];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int e = 4 * tid + t;
+        for (int t = 0; t < EPT; ++t) {
+            const int e = EPT * tid + t;
             valid[t] = e < size && fr.x + e < n;
             int4 mt = make_int4(-1, -1, 0, 0);
             p[t] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -356,8 +360,8 @@ __global__ __launch_bounds__(kFrameThreads) __attribute__((amdgpu_waves_per_eu(4
             eb[t] = valid[t] ? min(max(mt.x, 0), n - 1) : -1;
             ek[t] = valid[t] ? min(max(mt.y, 0), 8 * d.cap - 1) : -1;
             lv0[t] = min(max(mt.z, 0), kMaxLevel);
-            L.key[4 * tid + t] = ek[t];
-            if (valid[t]) L.body[4 * tid + t] = eb[t];
+            L.key[EPT * tid + t] = ek[t];
+            if (valid[t]) L.body[EPT * tid + t] = eb[t];
         }
         if (tid == 0) {
             L.nc = 0;
@@ -366,11 +370,11 @@ __global__ __launch_bounds__(kFrameThreads) __attribute__((amdgpu_waves_per_eu(4
         __syncthreads();
         // bucket leaders (the first entry of each key) and each entry's leader: an inclusive
         // max-scan of (leader ? e : -1) over the frame (in the thread, the wave, the block)
-        bool lead[4], multi[4];
-        int li[4];
+        bool lead[EPT], multi[EPT];
+        int li[EPT];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int e = 4 * tid + t;
+        for (int t = 0; t < EPT; ++t) {
+            const int e = EPT * tid + t;
             const int prev = e > 0 ? L.key[e - 1] : -1;
             const int next = e + 1 < size ? L.key[e + 1] : -1;
             lead[t] = valid[t] && (e == 0 || prev != ek[t]);
@@ -378,7 +382,7 @@ __global__ __launch_bounds__(kFrameThreads) __attribute__((amdgpu_waves_per_eu(4
             li[t] = lead[t] ? e : -1;
             if (t) li[t] = max(li[t], li[t - 1]);
         }
-        int scan = li[3];
+        int scan = li[EPT - 1];
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
             const int y = __shfl_up(scan, o);
@@ -390,11 +394,11 @@ __global__ __launch_bounds__(kFrameThreads) __attribute__((amdgpu_waves_per_eu(4
         __syncthreads();
         for (int q = 0; q < wave; ++q) before = max(before, L.wmax[q]);
 #pragma unroll
-        for (int t = 0; t < 4; ++t) li[t] = max(li[t], before);
+        for (int t = 0; t < EPT; ++t) li[t] = max(li[t], before);
         // each entry's box down to its bucket's slot of T (the level of the slot's cell came
         // from k_classify); the bucket leaders open the bucket roots
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
+        for (int t = 0; t < EPT; ++t) {
             act[t] = false;
             lvl[t] = lv0[t] + 1;
             if (!valid[t]) continue;
@@ -408,21 +412,25 @@ __global__ __launch_bounds__(kFrameThreads) __attribute__((amdgpu_waves_per_eu(4
                 for (int k = 0; k < 8; ++k) L.c[8 * c + k] = kEmpty;
                 L.par[c] = -1 - ek[t];
                 L.cont[c] = (lv0[t] + 1) << 8;
-                L.key[4 * tid + t] = c;  // read only by this bucket's entries from here on
+                L.key[EPT * tid + t] = c;  // read only by this bucket's entries from here on
             }
         }
         __syncthreads();
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+        for (int t = 0; t < EPT; ++t)
             if (multi[t]) {
                 node[t] = min(max(L.key[min(max(li[t], 0), kFrame - 1)], 0), kFrameCells - 1);
                 act[t] = true;
             }
         __syncthreads();
         // level by level, every bucket's subtree at once
-        while (__syncthreads_or(act[0] || act[1] || act[2] || act[3]) && !L.fail) {
+        while (true) {
+            bool any = false;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
+            for (int t = 0; t < EPT; ++t) any = any || act[t];
+            if (!__syncthreads_or(any) || L.fail) break;
+#pragma unroll
+            for (int t = 0; t < EPT; ++t) {
                 if (!act[t]) continue;
                 if (lvl[t] > kMaxLevel) {
                     L.fail = 1;  // the depth limit: the insertion reports it
@@ -434,16 +442,16 @@ __global__ __launch_bounds__(kFrameThreads) __attribute__((amdgpu_waves_per_eu(4
                 bx[t][0] = lx; bx[t][1] = ly; bx[t][2] = lz; bx[t][3] = hx; bx[t][4] = hy; bx[t][5] = hz;
                 dgs[t] = dg;
                 slot[t] = 8 * node[t] + dg;
-                if (atomicCAS(&L.c[slot[t]], kEmpty, 4 * tid + t) != kEmpty) atomicOr(&L.cont[node[t]], 1 << dg);
+                if (atomicCAS(&L.c[slot[t]], kEmpty, EPT * tid + t) != kEmpty) atomicOr(&L.cont[node[t]], 1 << dg);
             }
             __syncthreads();
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
+            for (int t = 0; t < EPT; ++t) {
                 if (!act[t]) continue;
                 if (!((L.cont[node[t]] >> dgs[t]) & 1)) {  // alone in its slot: placed
                     act[t] = false;
                     depth = max(depth, lvl[t]);
-                } else if (L.c[slot[t]] == 4 * tid + t) {  // the holder of a contested slot opens a cell
+                } else if (L.c[slot[t]] == EPT * tid + t) {  // the holder of a contested slot opens a cell
                     const int c = atomicAdd(&L.nc, 1);
                     if (c >= kFrameCells) {
                         L.fail = 1;
@@ -458,7 +466,7 @@ __global__ __launch_bounds__(kFrameThreads) __attribute__((amdgpu_waves_per_eu(4
             __syncthreads();
             if (L.fail) break;  // uniform: read between two barriers
 #pragma unroll
-            for (int t = 0; t < 4; ++t)
+            for (int t = 0; t < EPT; ++t)
                 if (act[t]) {  // into the new cell, one level down
                     node[t] = min(max(L.c[slot[t]] - kCellTag, 0), kFrameCells - 1);
                     ++lvl[t];
@@ -471,7 +479,7 @@ __global__ __launch_bounds__(kFrameThreads) __attribute__((amdgpu_waves_per_eu(4
             // nothing of this frame was written: its bodies go to the insertion, except the
             // ones already in their slots of T
 #pragma unroll
-            for (int t = 0; t < 4; ++t) to_fallback(valid[t] && C[ek[t]] != eb[t], eb[t], S, fblist + d.body_off);
+            for (int t = 0; t < EPT; ++t) to_fallback(valid[t] && C[ek[t]] != eb[t], eb[t], S, fblist + d.body_off);
             __syncthreads();
             continue;
         }
@@ -485,7 +493,7 @@ __global__ __launch_bounds__(kFrameThreads) __attribute__((amdgpu_waves_per_eu(4
             continue;
         }
         // the cells: slots, parent links, counts of child cells, cleared per-slot counts
-        for (int c = tid; c < nc; c += kFrameThreads) {
+        for (int c = tid; c < nc; c += kThreads) {
             const int g = base + c;
             int v[8], np = 0;
 #pragma unroll
@@ -507,12 +515,12 @@ __global__ __launch_bounds__(kFrameThreads) __attribute__((amdgpu_waves_per_eu(4
             }
         }
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
+        for (int t = 0; t < EPT; ++t) {
             if (!valid[t]) continue;
             if (multi[t]) {  // placed in a new cell: the body's entry of that cell's child block
                 CD[8 * (size_t)(base + node[t]) + dgs[t]] = p[t];
                 if (lead[t]) {  // the subtree takes the bucket's slot of T
-                    C[ek[t]] = n + base + L.key[4 * tid + t];
+                    C[ek[t]] = n + base + L.key[EPT * tid + t];
                     atomicAdd(cpend + d.cell_off + (ek[t] >> 3), 1);
                 }
             } else {  // alone in its bucket: straight into the slot
@@ -529,7 +537,7 @@ __global__ __launch_bounds__(kFrameThreads) __attribute__((amdgpu_waves_per_eu(4
             // LDS slots, read by nothing else once the cell is done, for its parent's round.
             __syncthreads();  // every cell's slots are written out (above)
             int lmax = 0, lmin = kMaxLevel + 1;
-            for (int c = tid; c < nc; c += kFrameThreads) {
+            for (int c = tid; c < nc; c += kThreads) {
                 const int lv = L.cont[c] >> 8;
                 lmax = max(lmax, lv);
                 lmin = min(lmin, lv);
@@ -541,12 +549,14 @@ __global__ __launch_bounds__(kFrameThreads) __attribute__((amdgpu_waves_per_eu(4
                 L.wmin[wave] = lmin;
             }
             __syncthreads();
-            lmax = max(L.wmax[0], L.wmax[1]);
-            lmin = min(L.wmin[0], L.wmin[1]);
+            for (int q = 0; q < kThreads / 64; ++q) {
+                lmax = max(lmax, L.wmax[q]);
+                lmin = min(lmin, L.wmin[q]);
+            }
             const float4* FP = lpos + d.body_off + fr.x;
             int* SO = slotoff + 8 * (size_t)d.cell_off;
             for (int lv = lmax; lv >= lmin; --lv) {  // uniform
-                for (int c = tid; c < nc; c += kFrameThreads) {
+                for (int c = tid; c < nc; c += kThreads) {
                     if ((L.cont[c] >> 8) != lv) continue;
                     int sv[8];
 #pragma unroll
@@ -628,9 +638,13 @@ void launch_frame_build(hipStream_t s, int nblk, const SysDesc* sd, SysState* st
                         float4* node4, int* child, int* cparent, int* cpend, int* ccount, int* slotoff,
                         float4* cdata, const float4* lpos, const int4* lmeta, const int2* bframe, int* fblist,
                         bool last) {
-    if (nblk > 0)
-        hipLaunchKernelGGL(k_frame_build, dim3(nblk), dim3(kFrameThreads), 0, s, sd, st, work, node4, child,
-                           cparent, cpend, ccount, slotoff, cdata, lpos, lmeta, bframe, fblist, last ? 1 : 0);
+    if (nblk <= 0) return;
+    if (last)
+        hipLaunchKernelGGL(k_frame_build<4>, dim3(nblk), dim3(kFrame / 4), 0, s, sd, st, work, node4, child, cparent,
+                           cpend, ccount, slotoff, cdata, lpos, lmeta, bframe, fblist, 1);
+    else
+        hipLaunchKernelGGL(k_frame_build<1>, dim3(nblk), dim3(kFrame), 0, s, sd, st, work, node4, child, cparent,
+                           cpend, ccount, slotoff, cdata, lpos, lmeta, bframe, fblist, 0);
 }
 
 }  // namespace nb

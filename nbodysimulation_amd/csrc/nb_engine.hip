@@ -407,8 +407,12 @@ void Engine::build() {
                       lpos, lmeta, bframe);
         launch_scatter(stream, tab_len[bt], d_desc, d_state, tab[bt], ppos, kbase, sorted, forder, ccount, sorder,
                        lpos, lmeta, fblist, stride, exclude);
-        launch_frame_build(stream, tab_len[T_FRAMES], d_desc, d_state, tab[T_FRAMES], node4, child, cparent, cpend,
-                           ccount, slotoff, cdata, lpos, lmeta, bframe, fblist, ph == kBuildPhases - 1);
+        // the last phase's frames over one workgroup per 256 bodies; a strided phase's few
+        // frames over the keys table (one workgroup per 4096 bodies)
+        const bool last = ph == kBuildPhases - 1;
+        const Table ft = last ? T_FRAMES : T_KEYS;
+        launch_frame_build(stream, tab_len[ft], d_desc, d_state, tab[ft], node4, child, cparent, cpend, ccount,
+                           slotoff, cdata, lpos, lmeta, bframe, fblist, last);
     }
     // the bodies every phase left (the fallback list: a frame over the LDS pool or the depth
     // limit, a bucket over the frame budget, a first phase over the LDS cell pool), inserted
