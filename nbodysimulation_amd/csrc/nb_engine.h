@@ -21,21 +21,18 @@ namespace nb {
 // kernel launchers (nb_kernels.hip)
 void launch_bounds(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, const BlkWork* work,
                    const float4* node4, float* partial);
-void launch_tree_clear(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
-                       int* child, int* cpend);
-// the bounds pass (nbounds workgroups of table wb) and the tree clear (ncells of table wc) in
-// one launch; the root's re-seeding is left to k_build_top
-// og/olen/keys/oout: the force pass's dispatch order (as launch_force_order) computed by one
-// more workgroup when olen is in (0, kOrderSmall]; otherwise not computed
-void launch_bounds_clear(hipStream_t s, int nbounds, const BlkWork* wb, int ncells, const BlkWork* wc,
-                         const SysDesc* sd, SysState* st, const float4* node4, float* partial, int* child,
-                         int* cpend, const BlkWork* og, int olen, int* keys, BlkWork* oout);
+// the bounds pass (nbounds workgroups of table wb) in one launch with the force pass's dispatch
+// order (as launch_force_order) computed by one more workgroup when olen is in (0, kOrderSmall]
+// (otherwise not computed); the root's re-seeding is left to k_build_top
+void launch_bounds_order(hipStream_t s, int nbounds, const BlkWork* wb, const SysDesc* sd, SysState* st,
+                         const float4* node4, float* partial, const BlkWork* og, int olen, int* keys, BlkWork* oout);
 // one system's per-cell (COM, mass) (cells: its node4 past the bodies) and body counts, for the
 // exports (the step writes them for the root only)
 void launch_cell_export(hipStream_t s, int n, int ncells, const int* child, const float4* cdata,
                         const int* slotoff, float4* cells, int* ccount);
-// slots NULL: 0..nsys-1; sd and slotoff NULL: the roots' per-slot counts are left alone
-void launch_tree_init(hipStream_t s, int nsys, const int* slots, const SysDesc* sd, SysState* st, int* slotoff);
+// slots NULL: 0..nsys-1; sd and slotoff NULL: the roots' slots and counts are left alone
+void launch_tree_init(hipStream_t s, int nsys, const int* slots, const SysDesc* sd, SysState* st, int* slotoff,
+                      int* child, int* cpend);
 // fblist: the build's fallback list, which takes the phase's bodies when the LDS tree overflows
 void launch_build_top(hipStream_t s, int nsys, const int* slots, const SysDesc* sd, SysState* st,
                       const float4* node4, int* child, int* cparent, int* cpend, float4* cdata, int* slotoff,
@@ -201,9 +198,9 @@ struct Engine {
 
     // whole-batch stage launches (asynchronous on `stream`)
     void bounds();
-    void tree_reset();
-    // bounds() and tree_reset() of a step in one launch, the root's re-seeding done by
-    // build()'s first kernel: only where build() follows at once
+    void tree_reset();  // an empty root per system (no pass clears the other cells: their creators write them)
+    // bounds() of a step with the force pass's dispatch order in one launch, the root's
+    // re-seeding (tree_reset()) done by build()'s first kernel: only where build() follows at once
     void bounds_reset();
     void build();
     void summarize();

@@ -197,7 +197,7 @@ nb_status Engine::reserve(const Capacity& c, int law_, bool kahan_, void* user_s
     NB_HIP(hipMemsetAsync(slotoff, 0, sizeof(int) * 8 * nc, stream));
     NB_HIP(hipMemsetAsync(d_state, 0, sizeof(SysState) * S, stream));
     NB_HIP(hipMemsetAsync(gkey, 0, sizeof(int) * (size_t)std::max(tab_cap[T_GROUPS], 1), stream));
-    launch_tree_init(stream, S, nullptr, nullptr, d_state, nullptr);  // slotoff is cleared above
+    launch_tree_init(stream, S, nullptr, nullptr, d_state, nullptr, nullptr, nullptr);  // slotoff is cleared above
     NB_HIP(hipGetLastError());
     NB_HIP(hipStreamSynchronize(stream));
     return NB_OK;
@@ -375,15 +375,13 @@ void Engine::bounds() {
 void Engine::bounds_reset() {
     TraceRange tr("nb:bounds_reset");
     const bool ord = lpt && tab_len[T_GROUPS] <= kOrderSmall;
-    launch_bounds_clear(stream, tab_len[T_BOUNDS], tab[T_BOUNDS], tab_len[T_CELLS], tab[T_CELLS], d_desc, d_state,
-                        node4, d_partial, child, cpend, tab[T_GROUPS], ord ? tab_len[T_GROUPS] : 0, gkey,
-                        tab[T_FORDER]);
+    launch_bounds_order(stream, tab_len[T_BOUNDS], tab[T_BOUNDS], d_desc, d_state, node4, d_partial, tab[T_GROUPS],
+                        ord ? tab_len[T_GROUPS] : 0, gkey, tab[T_FORDER]);
     order_done = ord;
 }
 void Engine::tree_reset() {
     TraceRange tr("nb:tree_reset");
-    launch_tree_clear(stream, tab_len[T_CELLS], d_desc, d_state, tab[T_CELLS], child, cpend);
-    launch_tree_init(stream, nactive, d_slots, d_desc, d_state, slotoff);
+    launch_tree_init(stream, nactive, d_slots, d_desc, d_state, slotoff, child, cpend);
 }
 // The first phase (every 4096th body) in LDS (k_build_top), or by insertion in global memory
 // for a system with more first-phase bodies than the LDS build takes; each later phase

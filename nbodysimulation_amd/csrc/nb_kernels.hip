@@ -20,11 +20,11 @@
 //
 // Memory-model notes (gfx950 has one L2 per XCD):
 //   * Data that one workgroup writes and another reads inside the same kernel is
-//     accessed with agent-scope atomics (sc1 loads/stores).  The tree build relies
-//     only on compare-and-swap for correctness: a stale read of a child slot can
-//     only return an older value of that slot, which makes the CAS fail and the
-//     body retry.  Slots of unused cells are -1 before a build starts, so a stale
-//     read of a freshly published cell can only be -1.
+//     accessed with agent-scope atomics (sc1 loads/stores).  The insertion kernel
+//     reads child slots that way and relies on compare-and-swap for correctness: a
+//     read that races a writer returns an older value of that slot, which makes the
+//     CAS fail and the body retry.  No pass clears the cell pool between steps: a
+//     cell's creator writes all eight slots before the cell becomes reachable.
 //   * A lock holder publishes a new subtree with one release store.
 //   * No lane ever waits inside an iteration for another lane of its own wave:
 //     every retry loop makes one attempt per iteration.
@@ -146,34 +146,19 @@ __global__ __launch_bounds__(256) void k_bounds(const SysDesc* __restrict__ sd, 
 // reset: clear the slots of the cells the previous build used (so every unused
 // cell holds -1 slots), then re-seed the root.
 
-__device__ __forceinline__ void tree_clear_block(const SysDesc* __restrict__ sd, const SysState* __restrict__ st,
-                                                 const BlkWork w, int* child, int* cpend) {
-    const SysDesc d = sd[w.sys];
-    const int used = min(st[w.sys].ncells, d.cap);
-    const int ci = w.begin + threadIdx.x;
-    if (ci >= w.end || ci >= used) return;
-    int4* c4 = reinterpret_cast<int4*>(child + 8 * ((size_t)d.cell_off + ci));
-    const int4 m1 = make_int4(-1, -1, -1, -1);
-    c4[0] = m1;
-    c4[1] = m1;
-    cpend[d.cell_off + ci] = 0;
-}
-
-__global__ __launch_bounds__(256) void k_tree_clear(const SysDesc* __restrict__ sd, const SysState* __restrict__ st,
-                                                    const BlkWork* __restrict__ work, int* child, int* cpend) {
-    tree_clear_block(sd, st, work[blockIdx.x], child, cpend);
-}
-
-
 __global__ void k_tree_init(int nsys, const int* __restrict__ slots, const SysDesc* __restrict__ sd, SysState* st,
-                            int* slotoff) {
+                            int* slotoff, int* child, int* cpend) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nsys) return;
     const int s = slots ? slots[k] : k;
-    if (sd && slotoff && sd[s].algo == 0 && sd[s].cap > 0) {  // the root's per-slot counts start at zero
+    if (sd && slotoff && sd[s].algo == 0 && sd[s].cap > 0) {  // an empty root: slots, counts
         int4* so = reinterpret_cast<int4*>(slotoff + 8 * (size_t)sd[s].cell_off);
         so[0] = make_int4(0, 0, 0, 0);
         so[1] = make_int4(0, 0, 0, 0);
+        int4* c4 = reinterpret_cast<int4*>(child + 8 * (size_t)sd[s].cell_off);
+        c4[0] = make_int4(kEmpty, kEmpty, kEmpty, kEmpty);
+        c4[1] = make_int4(kEmpty, kEmpty, kEmpty, kEmpty);
+        cpend[sd[s].cell_off] = 0;
     }
     st[s].ncells = 1;
     st[s].max_depth = 1;  // build_tree.cl:28
@@ -198,13 +183,11 @@ __global__ void k_tree_init(int nsys, const int* __restrict__ slots, const SysDe
 // then every 64th, then the rest): the top levels of the tree form with little
 // contention, later phases descend through them.  The topology does not depend
 // on insertion order.
-// The descent reads slots with L2-cacheable loads.  A cached value can only be
-// an older value of that slot (-1, a body, or -2; a cell pointer is final),
-// which makes the CAS fail; the retry then re-reads the slot coherently.
-
-__device__ __forceinline__ int ld_w(const int* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
+// The descent reads slots coherently: no pass clears the cell pool between steps (every
+// cell's creator writes all its slots), so a cached copy of a line that holds a cell
+// created during this launch could still carry the previous step's pointers.  This
+// insertion serves only the bodies the LDS and bucketed builds leave (usually none) and
+// the first phase of systems too large for the LDS build.
 
 __device__ __forceinline__ int wave_incl_scan(int x) {
     const int lane = threadIdx.x & 63;
@@ -251,7 +234,6 @@ __global__ __launch_bounds__(BLK) void k_build(const SysDesc* __restrict__ sd, S
     int b = 0, node = 0, lvl = 0, dg = 0;
     int depth = 1;
     int tries = 0;
-    bool fresh = false;  // re-read the current slot coherently
     bool active = false;
     auto start = [&](int body) {
         b = body;
@@ -262,7 +244,6 @@ __global__ __launch_bounds__(BLK) void k_build(const SysDesc* __restrict__ sd, S
         lvl = 0;
         dg = level_digit(d.rules, 0, px, py, pz, lx, ly, lz, hx, hy, hz);
         tries = 0;
-        fresh = false;
         active = true;
     };
     constexpr int kWaves = BLK / 64;
@@ -305,15 +286,14 @@ __global__ __launch_bounds__(BLK) void k_build(const SysDesc* __restrict__ sd, S
                 active = false;
             } else {
                 slot = 8 * node + dg;
-                v = fresh ? ld_c(C + slot) : ld_w(C + slot);
+                v = ld_c(C + slot);
                 while (v >= n) {
                     node = v - n;
                     ++lvl;
                     dg = level_digit(d.rules, lvl, px, py, pz, lx, ly, lz, hx, hy, hz);
                     slot = 8 * node + dg;
-                    v = ld_w(C + slot);
+                    v = ld_c(C + slot);
                 }
-                fresh = true;
                 if (v == kEmpty) {
                     if (cas_c(C + slot, kEmpty, b)) {
                         CD[slot] = make_float4(px, py, pz, pm);  // the body's entry of the cell's child block
@@ -406,11 +386,15 @@ __global__ __launch_bounds__(BLK) void k_build(const SysDesc* __restrict__ sd, S
                     int4* so = reinterpret_cast<int4*>(slotoff + 8 * ((size_t)d.cell_off + c));
                     so[0] = make_int4(0, 0, 0, 0);
                     so[1] = make_int4(0, 0, 0, 0);
-                    if (i < k - 1) {
-                        st_c(C + 8 * c + db2, n + c + 1);
-                    } else {
-                        st_c(C + 8 * c + db2, b);
-                        st_c(C + 8 * c + dq2, v);
+                    // every slot of the new cell (a cell's creator writes all eight; nothing clears them)
+#pragma unroll
+                    for (int k8 = 0; k8 < 8; ++k8) {
+                        int sv = kEmpty;
+                        if (k8 == db2) sv = i < k - 1 ? n + c + 1 : b;
+                        else if (i == k - 1 && k8 == dq2) sv = v;
+                        st_c(C + 8 * c + k8, sv);
+                    }
+                    if (i == k - 1) {
                         lb = 8 * c + db2;
                         lq = 8 * c + dq2;
                     }
@@ -488,10 +472,14 @@ __global__ __launch_bounds__(kTopBodies) void k_build_top(int nsys, const int* _
         S->bq = 0;
         S->top = 0;
         S->nfb = 0;
-        if (d.algo == 0 && d.cap > 0) {
+        if (d.algo == 0 && d.cap > 0) {  // an empty root (written over below when this kernel builds the top)
             int4* so = reinterpret_cast<int4*>(slotoff + 8 * (size_t)d.cell_off);
             so[0] = make_int4(0, 0, 0, 0);
             so[1] = make_int4(0, 0, 0, 0);
+            int4* c4 = reinterpret_cast<int4*>(child + 8 * (size_t)d.cell_off);
+            c4[0] = make_int4(kEmpty, kEmpty, kEmpty, kEmpty);
+            c4[1] = make_int4(kEmpty, kEmpty, kEmpty, kEmpty);
+            cpend[d.cell_off] = 0;
         }
     }
     if (d.algo != 0 || n == 0 || nb > kTopBodies || S->status) return;
@@ -1080,16 +1068,15 @@ __device__ __forceinline__ void force_order_small(const BlkWork* __restrict__ in
     }
 }
 
-// A step's bounds pass and tree clear in one launch: with an order table (olen > 0),
-// workgroup 0 computes the force pass's longest-first dispatch order (force_order_small)
-// beside the others; of the rest, those below nb take the bounds table (their ticket counts
-// only them) and the others the cell table.  The root's re-seeding (k_tree_init's work) is
-// then done by k_build_top, the build's first launch.
-__global__ __launch_bounds__(256) void k_bounds_clear(const SysDesc* __restrict__ sd, SysState* st,
-                                                      const BlkWork* __restrict__ wb, int nb,
-                                                      const BlkWork* __restrict__ wc,
-                                                      const float4* __restrict__ node4, float* partial, int* child,
-                                                      int* cpend, const BlkWork* __restrict__ og, int olen, int* keys,
+// A step's bounds pass and the force pass's dispatch order in one launch: with an order
+// table (olen > 0), workgroup 0 computes the longest-first order (force_order_small) beside
+// the bounds workgroups (whose ticket counts only them).  No pass clears the tree: every
+// cell's creator writes all its slots, and the root's re-seeding (k_tree_init's work) is done
+// by k_build_top, the build's first launch.
+__global__ __launch_bounds__(256) void k_bounds_order(const SysDesc* __restrict__ sd, SysState* st,
+                                                      const BlkWork* __restrict__ wb,
+                                                      const float4* __restrict__ node4, float* partial,
+                                                      const BlkWork* __restrict__ og, int olen, int* keys,
                                                       BlkWork* __restrict__ oout) {
     int blk = blockIdx.x;
     if (olen > 0) {
@@ -1099,10 +1086,7 @@ __global__ __launch_bounds__(256) void k_bounds_clear(const SysDesc* __restrict_
         }
         --blk;
     }
-    if (blk < nb)
-        bounds_block(sd, st, wb[blk], blk, node4, partial);
-    else
-        tree_clear_block(sd, st, wc[blk - nb], child, cpend);
+    bounds_block(sd, st, wb[blk], blk, node4, partial);
 }
 
 // The per-cell (COM, mass) and body count of one system's tree for the exports, from the
@@ -1520,24 +1504,19 @@ void launch_bounds(hipStream_t s, int nblk, const SysDesc* sd, SysState* st, con
                    const float4* node4, float* partial) {
     NB_LAUNCH(k_bounds, nblk, 256, s, sd, st, work, node4, partial);
 }
-void launch_tree_clear(hipStream_t s, int nblk, const SysDesc* sd, const SysState* st, const BlkWork* work,
-                       int* child, int* cpend) {
-    NB_LAUNCH(k_tree_clear, nblk, 256, s, sd, st, work, child, cpend);
-}
-void launch_bounds_clear(hipStream_t s, int nbounds, const BlkWork* wb, int ncells, const BlkWork* wc,
-                         const SysDesc* sd, SysState* st, const float4* node4, float* partial, int* child,
-                         int* cpend, const BlkWork* og, int olen, int* keys, BlkWork* oout) {
+void launch_bounds_order(hipStream_t s, int nbounds, const BlkWork* wb, const SysDesc* sd, SysState* st,
+                         const float4* node4, float* partial, const BlkWork* og, int olen, int* keys, BlkWork* oout) {
     if (olen > kOrderSmall) olen = 0;  // the caller orders with k_force_order then
-    NB_LAUNCH(k_bounds_clear, (olen > 0 ? 1 : 0) + nbounds + ncells, 256, s, sd, st, wb, nbounds, wc, node4, partial,
-              child, cpend, og, olen, keys, oout);
+    NB_LAUNCH(k_bounds_order, (olen > 0 ? 1 : 0) + nbounds, 256, s, sd, st, wb, node4, partial, og, olen, keys, oout);
 }
 void launch_cell_export(hipStream_t s, int n, int ncells, const int* child, const float4* cdata,
                         const int* slotoff, float4* cells, int* ccount) {
     NB_LAUNCH(k_cell_export, (ncells + 255) / 256, 256, s, n, ncells, child, cdata, slotoff, cells, ccount);
 }
-void launch_tree_init(hipStream_t s, int nsys, const int* slots, const SysDesc* sd, SysState* st, int* slotoff) {
+void launch_tree_init(hipStream_t s, int nsys, const int* slots, const SysDesc* sd, SysState* st, int* slotoff,
+                      int* child, int* cpend) {
     if (nsys <= 0) return;
-    NB_LAUNCH(k_tree_init, (nsys + 255) / 256, 256, s, nsys, slots, sd, st, slotoff);
+    NB_LAUNCH(k_tree_init, (nsys + 255) / 256, 256, s, nsys, slots, sd, st, slotoff, child, cpend);
 }
 void launch_build_top(hipStream_t s, int nsys, const int* slots, const SysDesc* sd, SysState* st,
                       const float4* node4, int* child, int* cparent, int* cpend, float4* cdata, int* slotoff,
