@@ -53,13 +53,26 @@ __device__ __forceinline__ void bounds_block(const SysDesc* __restrict__ sd, Sys
     float lo0 = INFINITY, lo1 = INFINITY, lo2 = INFINITY;
     float hi0 = -INFINITY, hi1 = -INFINITY, hi2 = -INFINITY;
     float bad = 0.0f;
-    for (int i = w.begin + threadIdx.x; i < w.end; i += blockDim.x) {
-        float4 p = nodes[i];
-        if (finite3(p)) {
-            lo0 = fminf(lo0, p.x); lo1 = fminf(lo1, p.y); lo2 = fminf(lo2, p.z);
-            hi0 = fmaxf(hi0, p.x); hi1 = fmaxf(hi1, p.y); hi2 = fmaxf(hi2, p.z);
-        } else {
-            bad = 1.0f;
+    // eight positions per lane in flight at once (issued together, then reduced), not one
+    // load and its wait per body
+    constexpr int kPer = 8;
+    for (int i0 = w.begin + (int)threadIdx.x; i0 < w.end; i0 += kPer * (int)blockDim.x) {
+        float4 q[kPer];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int i = i0 + j * (int)blockDim.x;
+            q[j] = nodes[min(i, w.end - 1)];
+        }
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            if (i0 + j * (int)blockDim.x >= w.end) continue;
+            const float4 p = q[j];
+            if (finite3(p)) {
+                lo0 = fminf(lo0, p.x); lo1 = fminf(lo1, p.y); lo2 = fminf(lo2, p.z);
+                hi0 = fmaxf(hi0, p.x); hi1 = fmaxf(hi1, p.y); hi2 = fmaxf(hi2, p.z);
+            } else {
+                bad = 1.0f;
+            }
         }
     }
     lo0 = wave_min(lo0); lo1 = wave_min(lo1); lo2 = wave_min(lo2);
